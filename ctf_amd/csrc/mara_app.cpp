@@ -1,0 +1,364 @@
+/* Mara's Lua-facing application API (luaopen_Mara, Mara/mara.cpp:137-268) restated in
+ * C++ over the maragpu device context. Mara keeps one MaraApplication per process and
+ * every verb edits it; here the verbs edit a maragpu_config and the device context is
+ * (re)built on first use after a change. Errors that the Lua binding reports with
+ * luaL_error return MARA_E_* with a message in mara_last_error().
+ */
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "../../include/mara_host.h"
+#include "../../include/maragpu.h"
+
+namespace {
+
+struct App {
+  bool started = false;
+  bool fluid_rmhd = false;   // MaraApplication's default fluid is Euler (hydro.cpp:22)
+  bool have_domain = false;
+  bool have_godunov = false;
+  bool have_boundary = false;
+  bool have_riemann = false;
+  bool have_advance = false;
+  maragpu_config cfg;
+  maragpu_ctx *ctx = nullptr;
+  bool dirty = true;
+  double last_max_lambda = 0.0;
+  std::string err;
+
+  App()
+  {
+    std::memset(&cfg, 0, sizeof(cfg));
+    cfg.gamma = 1.4;                 // MaraApplication default eos, hydro.cpp:23
+    cfg.recon = MARAGPU_PLM;         // GodunovOperator::reconstruct_method default, hydro.cpp:70
+    cfg.theta = 2.0;                 // weno.c:10
+    cfg.IS = MARAGPU_JS96;           // weno.c:12
+    cfg.sz10A = 50.0;                // weno.c:11
+    cfg.riemann = MARAGPU_HLLC;
+    cfg.rk = 3;
+    cfg.precision = MARAGPU_FP64;
+    cfg.c2p = MARAGPU_C2P_PLAIN;
+    cfg.pfloor = -1.0;               // rmhd-c2p.c:54
+    cfg.ereset = 0;
+    cfg.preset = 1.0;                // GodunovOperator(), hydro.cpp:75-78
+    cfg.dreset = 1.0;
+  }
+};
+
+App *g_app = nullptr;
+std::string g_err;
+
+int fail(const std::string &msg, int code = MARAGPU_E_INVALID)
+{
+  g_err = msg;
+  return code;
+}
+
+App *app()
+{
+  return g_app;
+}
+
+int need_app()
+{
+  if (!g_app) return fail("[Mara] call Mara.start() first");
+  return 0;
+}
+
+void mark_dirty()
+{
+  if (g_app) g_app->dirty = true;
+}
+
+int ensure_ctx()
+{
+  App *a = app();
+  if (!a->have_domain) return fail("[Mara] need a domain to run this, use set_domain");
+  if (!a->fluid_rmhd) return fail("[Mara] this build runs the 'rmhd' fluid only; use set_fluid('rmhd')");
+  if (!a->have_godunov) return fail("[Mara] need a godunov operator, use set_godunov");
+  if (!a->have_riemann) return fail("[Mara] need a riemann solver, use set_riemann");
+  if (!a->have_advance) return fail("[Mara] need an advance option, use set_advance");
+  if (!a->have_boundary) return fail("[Mara] need boundary conditions, use set_boundary");
+  if (!a->dirty && a->ctx) return 0;
+  if (a->ctx) {
+    maragpu_destroy(a->ctx);
+    a->ctx = nullptr;
+  }
+  int rc = maragpu_create(&a->cfg, &a->ctx);
+  if (rc) return fail(std::string("[Mara] device context: ") + maragpu_strerror(rc), rc);
+  a->dirty = false;
+  return 0;
+}
+
+size_t zone_doubles(const App *a)
+{
+  size_t n = 8;
+  for (int d = 0; d < 3; ++d) n *= (size_t)(a->cfg.N[d] + 2 * a->cfg.Ng);
+  return n;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mara_last_error(void) { return g_err.c_str(); }
+
+int mara_start(void)
+{
+  delete g_app;
+  g_app = new App;
+  g_app->started = true;
+  return 0;
+}
+
+int mara_close(void)
+{
+  if (g_app) {
+    if (g_app->ctx) maragpu_destroy(g_app->ctx);
+    delete g_app;
+    g_app = nullptr;
+  }
+  return 0;
+}
+
+int mara_set_fluid(const char *key)
+{
+  if (int rc = need_app()) return rc;
+  if (!key) return fail("[Mara] no such fluid '(null)'");
+  if (std::strcmp(key, "rmhd") == 0) {
+    app()->fluid_rmhd = true;
+    mark_dirty();
+    return 0;
+  }
+  if (std::strcmp(key, "euler") == 0 || std::strcmp(key, "srhd") == 0)
+    return fail(std::string("[Mara] fluid '") + key + "' is outside this build (rmhd only)");
+  return fail(std::string("[Mara] no such fluid '") + key + "'");
+}
+
+int mara_set_eos(const char *key, double gamma)
+{
+  if (int rc = need_app()) return rc;
+  if (key && std::strcmp(key, "gamma-law") == 0) {
+    if (!(gamma > 1.0)) return fail("[Mara] gamma-law index must exceed 1");
+    app()->cfg.gamma = gamma;
+    mark_dirty();
+    return 0;
+  }
+  if (key && (std::strcmp(key, "gamma-law + polytrope") == 0 || std::strcmp(key, "shen") == 0 ||
+              std::strcmp(key, "tabulated") == 0))
+    return fail(std::string("[Mara] eos '") + key + "' is outside this build (gamma-law only)");
+  return fail(std::string("[Mara] no such eos '") + (key ? key : "(null)") + "'");
+}
+
+int mara_set_boundary(const char *key)
+{
+  if (int rc = need_app()) return rc;
+  if (key && std::strcmp(key, "periodic") == 0) {
+    app()->have_boundary = true;
+    return 0;
+  }
+  if (key && (std::strcmp(key, "outflow") == 0 || std::strcmp(key, "perxouty") == 0 ||
+              std::strcmp(key, "reflect2d") == 0 || std::strcmp(key, "magnetic-bubble") == 0 ||
+              std::strcmp(key, "wind-inflow") == 0))
+    return fail(std::string("[Mara] boundary '") + key + "' is outside this build (periodic only)");
+  return fail(std::string("[Mara] no such boundary conditions '") + (key ? key : "(null)") + "'");
+}
+
+int mara_set_riemann(const char *key)
+{
+  if (int rc = need_app()) return rc;
+  int r = -1;
+  if (key && std::strcmp(key, "hll") == 0) r = MARAGPU_HLL;
+  if (key && std::strcmp(key, "hllc") == 0) r = MARAGPU_HLLC;
+  if (key && std::strcmp(key, "hlld") == 0) r = MARAGPU_HLLD;
+  if (r < 0) return fail(std::string("[Mara] no such riemann solver '") + (key ? key : "(null)") + "'");
+  app()->cfg.riemann = r;
+  app()->have_riemann = true;
+  mark_dirty();
+  return 0;
+}
+
+int mara_set_godunov(const char *key)
+{
+  if (int rc = need_app()) return rc;
+  if (key && std::strcmp(key, "plm-split") == 0) {
+    app()->have_godunov = true;
+    mark_dirty();
+    return 0;
+  }
+  if (key && (std::strcmp(key, "plm-muscl") == 0 || std::strcmp(key, "weno-split") == 0))
+    return fail(std::string("[Mara] integration scheme '") + key +
+                "' is outside this build (plm-split with extrap pcm|plm|weno5)");
+  return fail(std::string("[Mara] no such integration scheme '") + (key ? key : "(null)") + "'");
+}
+
+int mara_set_advance(const char *key)
+{
+  if (int rc = need_app()) return rc;
+  int r = 0;
+  if (key && std::strcmp(key, "single") == 0) r = 1;
+  if (key && std::strcmp(key, "rk2") == 0) r = 2;
+  if (key && std::strcmp(key, "rk3") == 0) r = 3;
+  if (key && std::strcmp(key, "rk4") == 0) r = 4;
+  if (!r) return fail(std::string("[Mara] no such advance option '") + (key ? key : "(null)") + "'");
+  app()->cfg.rk = r;
+  app()->have_advance = true;
+  mark_dirty();
+  return 0;
+}
+
+/* luaC_config_solver, one table entry at a time; unknown keys are ignored as there */
+int mara_config_solver_str(const char *key, const char *value)
+{
+  if (int rc = need_app()) return rc;
+  if (!key || !value) return fail("[Mara] config_solver: null key or value");
+  maragpu_config &c = app()->cfg;
+  if (std::strcmp(key, "fsplit") == 0) {
+    if (std::strcmp(value, "llf") && std::strcmp(value, "marq"))
+      return fail(std::string("[Mara] no such fsplit '") + value + "'");
+    return 0;  // consumed by weno-split only
+  }
+  if (std::strcmp(key, "extrap") == 0) {
+    if (!std::strcmp(value, "pcm")) c.recon = MARAGPU_PCM;
+    else if (!std::strcmp(value, "plm")) c.recon = MARAGPU_PLM;
+    else if (!std::strcmp(value, "weno5")) c.recon = MARAGPU_WENO5;
+    else return fail(std::string("[Mara] no such extrap '") + value + "'");
+    mark_dirty();
+    return 0;
+  }
+  if (std::strcmp(key, "IS") == 0) {
+    if (!std::strcmp(value, "js96")) c.IS = MARAGPU_JS96;
+    else if (!std::strcmp(value, "b08")) c.IS = MARAGPU_B08;
+    else if (!std::strcmp(value, "sz10")) c.IS = MARAGPU_SZ10;
+    else return fail(std::string("no such IS: ") + value);
+    mark_dirty();
+    return 0;
+  }
+  if (std::strcmp(key, "precision") == 0) {
+    if (!std::strcmp(value, "fp64")) c.precision = MARAGPU_FP64;
+    else if (!std::strcmp(value, "mixed")) c.precision = MARAGPU_MIXED;
+    else return fail(std::string("[Mara] no such precision '") + value + "'");
+    mark_dirty();
+    return 0;
+  }
+  if (std::strcmp(key, "c2p") == 0) {
+    if (!std::strcmp(value, "plain")) c.c2p = MARAGPU_C2P_PLAIN;
+    else if (!std::strcmp(value, "learned")) c.c2p = MARAGPU_C2P_LEARNED;
+    else return fail(std::string("[Mara] no such c2p '") + value + "'");
+    mark_dirty();
+    return 0;
+  }
+  return 0;
+}
+
+int mara_config_solver_num(const char *key, double value)
+{
+  if (int rc = need_app()) return rc;
+  if (!key) return fail("[Mara] config_solver: null key");
+  maragpu_config &c = app()->cfg;
+  if (!std::strcmp(key, "theta")) c.theta = value;
+  else if (!std::strcmp(key, "sz10A")) c.sz10A = value;
+  else if (!std::strcmp(key, "pfloor")) c.pfloor = value;
+  else if (!std::strcmp(key, "preset")) c.preset = value;
+  else if (!std::strcmp(key, "dreset")) c.dreset = value;
+  else return 0;
+  mark_dirty();
+  return 0;
+}
+
+int mara_config_solver_bool(const char *key, int value)
+{
+  if (int rc = need_app()) return rc;
+  if (!key) return fail("[Mara] config_solver: null key");
+  if (!std::strcmp(key, "ereset")) {
+    app()->cfg.ereset = value ? 1 : 0;
+    mark_dirty();
+  }
+  return 0;
+}
+
+int mara_set_domain(const double *x0, const double *x1, const int *N, int Nd, int Nq, int Ng)
+{
+  if (int rc = need_app()) return rc;
+  if (!x0 || !x1 || !N) return fail("[Mara] set_domain: null argument");
+  if (Nd != 3) return fail("[Mara] this build runs 3D domains only");
+  if (Nq != 8) return fail("[Mara] rmhd needs Nq = 8");
+  maragpu_config &c = app()->cfg;
+  for (int d = 0; d < 3; ++d) {
+    if (N[d] < 1) return fail("[Mara] set_domain: N must be positive");
+    c.N[d] = N[d];
+    c.x0[d] = x0[d];
+    c.x1[d] = x1[d];
+  }
+  c.Ng = Ng;
+  app()->have_domain = true;
+  mark_dirty();
+  return 0;
+}
+
+int mara_advance(double *P, size_t n, double dt, double *kzps, int *nerr)
+{
+  if (int rc = need_app()) return rc;
+  if (int rc = ensure_ctx()) return rc;
+  App *a = app();
+  if (!P) return fail("[Mara] advance: null buffer");
+  if (n != zone_doubles(a)) return fail("[Mara] primitive has the wrong size for the domain", MARAGPU_E_SIZE);
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = maragpu_upload_prim(a->ctx, P, n);
+  if (rc) return fail(std::string("[Mara] upload: ") + maragpu_strerror(rc), rc);
+  maragpu_stats st;
+  const int errors = maragpu_advance(a->ctx, dt, &st);
+  if (errors < 0) return fail(std::string("[Mara] advance: ") + maragpu_strerror(errors), errors);
+  a->last_max_lambda = st.max_lambda;
+  if (errors == 0) {
+    rc = maragpu_download_prim(a->ctx, P, n);
+    if (rc) return fail(std::string("[Mara] download: ") + maragpu_strerror(rc), rc);
+  }
+  const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (kzps) *kzps = 1e-3 * (double)(n / 8) / sec;
+  if (nerr) *nerr = errors;
+  return 0;
+}
+
+double mara_get_timestep(double cfl)
+{
+  // CFL * min_dx / RiemannSolver::MaxLambda of the last step (inf before any solve)
+  if (!g_app || !g_app->have_domain) return 0.0 / 0.0;
+  double min_dx = g_app->cfg.x1[0] - g_app->cfg.x0[0];
+  for (int d = 0; d < 3; ++d) {
+    const double dx = (g_app->cfg.x1[d] - g_app->cfg.x0[d]) / g_app->cfg.N[d];
+    if (dx < min_dx) min_dx = dx;
+  }
+  return cfl * min_dx / g_app->last_max_lambda;
+}
+
+int mara_set_primitive(const double *P, size_t n)
+{
+  if (int rc = need_app()) return rc;
+  if (int rc = ensure_ctx()) return rc;
+  if (n != zone_doubles(app())) return fail("[Mara] primitive has the wrong size for the domain", MARAGPU_E_SIZE);
+  int rc = maragpu_upload_prim(app()->ctx, P, n);
+  if (rc) return fail(std::string("[Mara] upload: ") + maragpu_strerror(rc), rc);
+  return 0;
+}
+
+int mara_measure_mean_max_divB(double *mean, double *max)
+{
+  if (int rc = need_app()) return rc;
+  if (!app()->ctx) {
+    if (mean) *mean = 0.0;
+    if (max) *max = 0.0;
+    return 0;
+  }
+  return maragpu_measure_divB(app()->ctx, mean, max, nullptr);
+}
+
+struct maragpu_ctx *mara_device_context(void)
+{
+  if (!g_app) return nullptr;
+  if (ensure_ctx()) return nullptr;
+  return g_app->ctx;
+}
+
+}  // extern "C"

@@ -1,0 +1,1008 @@
+/* maragpu — device-resident 3D relativistic-MHD update for MI355X (gfx950).
+ *
+ * Data layout in HBM: structure of arrays, one plane of T doubles per field, T = all
+ * zones of the augmented (ghost-including) grid, flat zone index z = (i*n1 + j)*n2 + k
+ * as in Mara's AoS layout (Mara/valman.hpp:25-36) with the field index moved outward.
+ *   Pc   committed primitives (the user's state)   8 x T doubles
+ *   Pw   working primitives / c2p guesses          8 x T doubles
+ *   U0   conserved state at the start of the step  8 x T doubles
+ *   U    conserved state of the current RK stage   8 x T doubles
+ *   F[3] face fluxes per axis (double, or float in the mixed form)
+ *   A    RK4 accumulator (rk4 only)
+ *
+ * One step (luaC_Mara_advance, Mara/mara.cpp:297-341) is, per RK stage:
+ *   k_cons_to_prim    per interior zone: c2p cascade, result written to the zone and its
+ *                     periodic ghost images (replaces Synchronize + per-zone c2p)
+ *   k_flux<DIM> x 3   per interface over the reference's flat sweep range
+ *                     (Mara/plm-split.cpp:48): reconstruct 8 fields, Riemann solve,
+ *                     PCM retry, max |lambda|
+ *   k_update          per interior zone: constrained-transport EMF averages
+ *                     (Mara/rmhd.cpp:293-337), flux divergence (plm-split.cpp:117-119) and
+ *                     the RK stage combination (Mara/runge-kutta.hpp), all in FP64
+ * followed by a final k_cons_to_prim; the state is committed only if no zone failed.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "rmhd_device.hpp"
+#include "../../include/maragpu.h"
+
+using namespace maragpu;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+/* Device-side counters of one step; reduced per block, then atomically. */
+struct DevStats {
+  unsigned long long lam_bits;         // max |lambda| as the bits of a non-negative double
+  unsigned long long hlld_success;
+  unsigned long long hlld_failure;
+  unsigned long long retries;
+  unsigned long long failures;
+  unsigned long long c2p_zones;
+  unsigned long long c2p_iters;
+  unsigned long long c2p_fallbacks;
+  unsigned long long hlld_c2p_iters;
+  unsigned long long hlld_newton_iters;
+  unsigned long long mlp_guesses;
+  unsigned long long mlp_fallbacks;
+  unsigned long long stage_failed[6];  // failed zones per c2p pass (ghost images included)
+};
+
+struct GridDesc {
+  int N[3], Ng, n[3];
+  long T;                  // zones including ghosts
+  long sx, sy, sz;         // zone strides
+  double dx, dy, dz, min_dx;
+};
+
+#define HIP_CHECK(expr)                                                            \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      std::fprintf(stderr, "[maragpu] %s failed: %s\n", #expr, hipGetErrorString(e_)); \
+      return MARAGPU_E_HIP;                                                        \
+    }                                                                              \
+  } while (0)
+
+/* --------------------------------------------------------------------------
+ * block reductions
+ * ------------------------------------------------------------------------ */
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+/* Sums NV counters over the block and adds them to dst[0..NV). */
+template <int NV>
+__device__ __forceinline__ void block_add(const unsigned long long (&v)[NV], unsigned long long *const (&dst)[NV])
+{
+  __shared__ unsigned long long red[kBlock / 64][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const unsigned long long s = wave_sum(v[i]);
+    if (lane == 0) red[wave][i] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    unsigned long long s = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    if (s) atomicAdd(dst[threadIdx.x], s);
+  }
+}
+
+__device__ __forceinline__ void block_max_lambda(double lam, unsigned long long *dst)
+{
+  __shared__ double red[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // NaN never enters: fmax drops it, as `if (MaxLambda < ml)` does in the reference
+  const double m = wave_max(lam);
+  if (lane == 0) red[wave] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) b = fmax(b, red[w]);
+    if (b > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(b));
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * layout conversion and closed forms
+ * ------------------------------------------------------------------------ */
+__global__ void k_aos_to_soa(const double *__restrict__ aos, double *__restrict__ soa, long T)
+{
+  const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= T) return;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) soa[q * T + z] = aos[8 * z + q];
+}
+
+__global__ void k_soa_to_aos(const double *__restrict__ soa, double *__restrict__ aos, long T)
+{
+  const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= T) return;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) aos[8 * z + q] = soa[q * T + z];
+}
+
+__device__ __forceinline__ long interior_zone(const GridDesc &g, long t)
+{
+  const long nzk = g.N[2], nyk = g.N[1];
+  const long k = t % nzk;
+  const long j = (t / nzk) % nyk;
+  const long i = t / (nzk * nyk);
+  return ((i + g.Ng) * g.n[1] + (j + g.Ng)) * g.n[2] + (k + g.Ng);
+}
+
+/* GodunovOperator::PrimToCons over the interior (Mara/hydro.cpp:112-123); U0 and U */
+__global__ void k_prim_to_cons(const double *__restrict__ P, double *__restrict__ U0,
+                               double *__restrict__ U, GridDesc g, double gamma)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  if (t >= ni) return;
+  const long z = interior_zone(g, t);
+  const long T = g.T;
+  double p[8], u[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) p[q] = P[q * T + z];
+  prim_to_cons(p, u, gamma - 1.0);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    U0[q * T + z] = u[q];
+    U[q * T + z] = u[q];
+  }
+}
+
+/* Writes the 8 fields of interior zone (i,j,k) to the zone and to each periodic ghost
+ * image of it: after this every ghost holds its image, which is what
+ * SimpleCartesianDomain::Synchronize produces (Mara/simple-cart.cpp:81-170). Returns the
+ * number of copies (the zone's multiplicity in Mara's all-zone loops). */
+__device__ __forceinline__ int scatter_images(const GridDesc &g, double *__restrict__ P, long T,
+                                              int i, int j, int k, const double *v)
+{
+  int li[3], lj[3], lk[3];
+  int ni = 0, nj = 0, nk = 0;
+  li[ni++] = i; if (i >= g.N[0]) li[ni++] = i - g.N[0]; if (i < 2 * g.Ng) li[ni++] = i + g.N[0];
+  lj[nj++] = j; if (j >= g.N[1]) lj[nj++] = j - g.N[1]; if (j < 2 * g.Ng) lj[nj++] = j + g.N[1];
+  lk[nk++] = k; if (k >= g.N[2]) lk[nk++] = k - g.N[2]; if (k < 2 * g.Ng) lk[nk++] = k + g.N[2];
+  for (int a = 0; a < ni; ++a)
+    for (int b = 0; b < nj; ++b)
+      for (int c = 0; c < nk; ++c) {
+        const long z = ((long)li[a] * g.n[1] + lj[b]) * g.n[2] + lk[c];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) P[q * T + z] = v[q];
+      }
+  return ni * nj * nk;
+}
+
+/* Per-zone conserved->primitive (GodunovOperator::ConsToPrim, Mara/hydro.cpp:125-202)
+ * with the emergency reset of :165-196. `first` = the previous primitive state is the
+ * zero-filled PrimitiveArray (first RK stage). */
+__global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restrict__ U,
+                                                         double *__restrict__ P, GridDesc g,
+                                                         double gamma, double pfloor, int first,
+                                                         int ereset, double preset, double dreset,
+                                                         DevStats *st, int pass)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  if (t < ni) {
+    const long T = g.T;
+    const int k = (int)(t % g.N[2]) + g.Ng;
+    const int j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+    const int i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+    const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
+    double u[8], p[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      u[q] = U[q * T + z];
+      p[q] = first ? 0.0 : P[q * T + z];
+    }
+    int iters = 0, which = 0;
+    const int err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
+    int failed = err != 0;
+    if (failed && ereset) {
+      p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
+      p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
+      failed = 0;
+    }
+    const int mult = scatter_images(g, P, T, i, j, k, p);
+    cnt[0] = 1;
+    cnt[1] = (unsigned long long)iters;
+    cnt[2] = which != 0;
+    cnt[3] = failed ? (unsigned long long)mult : 0ull;
+  }
+  unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                       &st->stage_failed[pass] };
+  block_add<4>(cnt, dst);
+}
+
+/* --------------------------------------------------------------------------
+ * Flux sweep along axis DIM (MethodOfLinesSplit::intercell_flux_sweep,
+ * Mara/plm-split.cpp:43-74): one thread per interface z | z+S over the reference's
+ * flat range z in [2S, T-3S), S the zone stride of the axis.
+ * ------------------------------------------------------------------------ */
+template <typename R, int DIM>
+__device__ __forceinline__ int riemann_dispatch(int solver, const R *pl, const R *pr, R *F, R gamma,
+                                                double pfloor, R &lam, FluxCounters &cnt)
+{
+  if (solver == RIEMANN_HLLD) return riemann_hlld<R, DIM>(pl, pr, F, gamma, pfloor, lam, cnt);
+  if (solver == RIEMANN_HLLC) return riemann_hllc<R, DIM>(pl, pr, F, gamma, lam);
+  return riemann_hll<R, DIM>(pl, pr, F, gamma, lam);
+}
+
+template <typename R, typename FT, int DIM, int SOLVER>
+__global__ __launch_bounds__(kBlock) void k_flux(const double *__restrict__ P, FT *__restrict__ Fout,
+                                                 GridDesc g, int recon, PhysParams pp, DevStats *st)
+{
+  const long T = g.T;
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  const long z = 2 * S + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double lam_out = 0.0;
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  if (z < T - 3 * S) {
+    R pl[8], pr[8], p0[8], p1[8];
+    const R theta = (R)pp.theta;
+    for (int q = 0; q < 8; ++q) {
+      const double *Pq = P + q * T + z;
+      const R v0 = (R)Pq[0], v1 = (R)Pq[S];
+      p0[q] = v0;
+      p1[q] = v1;
+      if (recon == RECON_WENO5) {
+        const R vm2 = (R)Pq[-2 * S], vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S], v3 = (R)Pq[3 * S];
+        pl[q] = weno5<R, true>(vm2, vm1, v0, v1, v2, pp.IS, (R)pp.sz10A);
+        pr[q] = weno5<R, false>(vm1, v0, v1, v2, v3, pp.IS, (R)pp.sz10A);
+      }
+      else if (recon == RECON_PLM) {
+        const R vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S];
+        pl[q] = plm<R>(vm1, v0, v1, R(+1.0), theta);
+        pr[q] = plm<R>(v0, v1, v2, R(-1.0), theta);
+      }
+      else {
+        pl[q] = v0;
+        pr[q] = v1;
+      }
+    }
+    R lam = R(0);
+    FluxCounters fc = {0, 0, 0, 0};
+    R F[8];
+    int err = riemann_dispatch<R, DIM>(SOLVER, pl, pr, F, (R)pp.gamma, pp.pfloor, lam, fc);
+    if (err) {
+      cnt[2] = 1;
+      err = riemann_dispatch<R, DIM>(SOLVER, p0, p1, F, (R)pp.gamma, pp.pfloor, lam, fc);
+      if (err) {
+        cnt[3] = 1;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) F[q] = R(0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Fout[q * T + z] = (FT)F[q];
+    lam_out = (double)lam;
+    cnt[0] = (unsigned long long)fc.hlld_ok;
+    cnt[1] = (unsigned long long)fc.hlld_fail;
+    cnt[4] = (unsigned long long)fc.c2p_iters;
+    cnt[5] = (unsigned long long)fc.newton_iters;
+  }
+  block_max_lambda(lam_out, &st->lam_bits);
+  unsigned long long *const dst[6] = { &st->hlld_success, &st->hlld_failure, &st->retries,
+                                       &st->failures, &st->hlld_c2p_iters, &st->hlld_newton_iters };
+  block_add<6>(cnt, dst);
+}
+
+/* --------------------------------------------------------------------------
+ * CT + divergence + RK stage (FP64, no contraction: the same roundings as the reference)
+ * ------------------------------------------------------------------------ */
+enum StageKind {
+  ST_EULER = 0,      // U <- U0 + dt L            (single; rk2/rk3 stage 1)
+  ST_RK2_FINAL,      // U <- 0.5 (U0 + U + dt L)
+  ST_RK3_2,          // U <- 3/4 U0 + 1/4 U + 1/4 dt L
+  ST_RK3_3,          // U <- 1/3 U0 + 2/3 U + 2/3 dt L
+  ST_RK4_1,          // A <- dt L;          U <- U0 + 0.5 A
+  ST_RK4_2,          // A <- A + 2 dt L;    U <- U0 + 0.5 dt L
+  ST_RK4_3,          // A <- A + 2 dt L;    U <- U0 + 1.0 dt L
+  ST_RK4_4           // U <- U0 + 1/6 (A + dt L)
+};
+
+template <typename FT>
+__device__ __forceinline__ double ld(const FT *__restrict__ a, long i) { return (double)a[i]; }
+
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
+                                                   const FT *__restrict__ Fz,
+                                                   const double *__restrict__ U0, double *__restrict__ U,
+                                                   double *__restrict__ A, GridDesc g, double dt, int kind)
+{
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  if (t >= ni) return;
+  const long z = interior_zone(g, t);
+  const long T = g.T, sx = g.sx, sy = g.sy, sz = g.sz;
+  const FT *FxBy = Fx + (long)By * T, *FxBz = Fx + (long)Bz * T;
+  const FT *FyBx = Fy + (long)Bx * T, *FyBz = Fy + (long)Bz * T;
+  const FT *FzBx = Fz + (long)Bx * T, *FzBy = Fz + (long)By * T;
+
+  // Rmhd::ConstrainedTransport3d, Mara/rmhd.cpp:310-329, at zone i
+  auto ct_xBy = [&](long i) {
+    return (2*ld(FxBy,i)+ld(FxBy,i+sy)+ld(FxBy,i-sy)-ld(FyBx,i)-ld(FyBx,i+sx)-ld(FyBx,i-sy)-ld(FyBx,i+sx-sy))*0.125;
+  };
+  auto ct_yBx = [&](long i) {
+    return (2*ld(FyBx,i)+ld(FyBx,i+sx)+ld(FyBx,i-sx)-ld(FxBy,i)-ld(FxBy,i+sy)-ld(FxBy,i-sx)-ld(FxBy,i-sx+sy))*0.125;
+  };
+  auto ct_yBz = [&](long i) {
+    return (2*ld(FyBz,i)+ld(FyBz,i+sz)+ld(FyBz,i-sz)-ld(FzBy,i)-ld(FzBy,i+sy)-ld(FzBy,i-sz)-ld(FzBy,i+sy-sz))*0.125;
+  };
+  auto ct_zBy = [&](long i) {
+    return (2*ld(FzBy,i)+ld(FzBy,i+sy)+ld(FzBy,i-sy)-ld(FyBz,i)-ld(FyBz,i+sz)-ld(FyBz,i-sy)-ld(FyBz,i-sy+sz))*0.125;
+  };
+  auto ct_zBx = [&](long i) {
+    return (2*ld(FzBx,i)+ld(FzBx,i+sx)+ld(FzBx,i-sx)-ld(FxBz,i)-ld(FxBz,i+sz)-ld(FxBz,i-sx)-ld(FxBz,i+sz-sx))*0.125;
+  };
+  auto ct_xBz = [&](long i) {
+    return (2*ld(FxBz,i)+ld(FxBz,i+sz)+ld(FxBz,i-sz)-ld(FzBx,i)-ld(FzBx,i+sx)-ld(FzBx,i-sz)-ld(FzBx,i-sz+sx))*0.125;
+  };
+
+  double F0[8], F1[8], G0[8], G1[8], H0[8], H1[8];  // fluxes at z and at z - s
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    F0[q] = ld(Fx + q * T, z); F1[q] = ld(Fx + q * T, z - sx);
+    G0[q] = ld(Fy + q * T, z); G1[q] = ld(Fy + q * T, z - sy);
+    H0[q] = ld(Fz + q * T, z); H1[q] = ld(Fz + q * T, z - sz);
+  }
+  F0[Bx] = 0.0;           F1[Bx] = 0.0;
+  F0[By] = ct_xBy(z);     F1[By] = ct_xBy(z - sx);
+  F0[Bz] = ct_xBz(z);     F1[Bz] = ct_xBz(z - sx);
+  G0[Bx] = ct_yBx(z);     G1[Bx] = ct_yBx(z - sy);
+  G0[By] = 0.0;           G1[By] = 0.0;
+  G0[Bz] = ct_yBz(z);     G1[Bz] = ct_yBz(z - sy);
+  H0[Bx] = ct_zBx(z);     H1[Bx] = ct_zBx(z - sz);
+  H0[By] = ct_zBy(z);     H1[By] = ct_zBy(z - sz);
+  H0[Bz] = 0.0;           H1[Bz] = 0.0;
+
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    // Mara/plm-split.cpp:117-119
+    const double L = -(F0[q]-F1[q])/g.dx - (G0[q]-G1[q])/g.dy - (H0[q]-H1[q])/g.dz;
+    const long m = q * T + z;
+    const double u0 = U0[m];
+    switch (kind) {
+    case ST_EULER: U[m] = u0 + dt*L; break;
+    case ST_RK2_FINAL: U[m] = 0.5*(u0 + U[m] + dt*L); break;
+    case ST_RK3_2: U[m] = 3./4*u0 + 1./4*U[m] + 1./4*dt*L; break;
+    case ST_RK3_3: U[m] = 1./3*u0 + 2./3*U[m] + 2./3*dt*L; break;
+    case ST_RK4_1: { const double l = dt*L; A[m] = l; U[m] = u0 + 0.5*l; } break;
+    case ST_RK4_2: { const double l = dt*L; A[m] = A[m] + 2.0*l; U[m] = u0 + 0.5*l; } break;
+    case ST_RK4_3: { const double l = dt*L; A[m] = A[m] + 2.0*l; U[m] = u0 + 1.0*l; } break;
+    default: { const double l = dt*L; U[m] = u0 + (1.0/6.0)*(A[m] + l); } break;
+    }
+  }
+}
+
+/* luaC_mean_max_divB stencil, Mara/mara.cpp:2298-2314, over its loop range */
+__global__ void k_divB(const double *__restrict__ P, GridDesc g, double *out /* sum, max, absmax */)
+{
+#pragma clang fp contract(off)
+  const long T = g.T;
+  const int m0 = g.n[0] - 2 * g.Ng - 1, m1 = g.n[1] - 2 * g.Ng - 1, m2 = g.n[2] - 2 * g.Ng - 1;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long cnt = (long)m0 * m1 * m2;
+  double d = 0.0;
+  if (t < cnt) {
+    const long k = t % m2 + g.Ng, j = (t / m2) % m1 + g.Ng, i = t / ((long)m2 * m1) + g.Ng;
+    auto M = [&](long a, long b, long c) { return (a * g.n[1] + b) * g.n[2] + c; };
+    const double *fx = P + (long)Bx * T, *fy = P + (long)By * T, *fz = P + (long)Bz * T;
+    d = ((fx[M(i+1,j  ,k  )] + fx[M(i+1,j+1,k  )] + fx[M(i+1,j  ,k+1)] + fx[M(i+1,j+1,k+1)]) -
+         (fx[M(i  ,j  ,k  )] + fx[M(i  ,j+1,k  )] + fx[M(i  ,j  ,k+1)] + fx[M(i  ,j+1,k+1)])) / 4.0
+      + ((fy[M(i  ,j+1,k  )] + fy[M(i  ,j+1,k+1)] + fy[M(i+1,j+1,k  )] + fy[M(i+1,j+1,k+1)]) -
+         (fy[M(i  ,j  ,k  )] + fy[M(i  ,j  ,k+1)] + fy[M(i+1,j  ,k  )] + fy[M(i+1,j  ,k+1)])) / 4.0
+      + ((fz[M(i  ,j  ,k+1)] + fz[M(i+1,j  ,k+1)] + fz[M(i  ,j+1,k+1)] + fz[M(i+1,j+1,k+1)]) -
+         (fz[M(i  ,j  ,k  )] + fz[M(i+1,j  ,k  )] + fz[M(i  ,j+1,k  )] + fz[M(i+1,j+1,k  )])) / 4.0;
+  }
+  __shared__ double rs[kBlock / 64], rm[kBlock / 64], ra[kBlock / 64];
+  double s = d, mx = (t < cnt) ? d : -INFINITY, ax = fabs(d);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    s += __shfl_xor(s, off, 64);
+    mx = fmax(mx, __shfl_xor(mx, off, 64));
+    ax = fmax(ax, __shfl_xor(ax, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { rs[wave] = s; rm[wave] = mx; ra[wave] = ax; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bs = 0.0, bm = -INFINITY, ba = 0.0;
+    for (int w = 0; w < kBlock / 64; ++w) { bs += rs[w]; bm = fmax(bm, rm[w]); ba = fmax(ba, ra[w]); }
+    atomicAdd(&out[0], bs);
+    // signed max via the order-preserving integer image of a double
+    long long bits = __double_as_longlong(bm);
+    bits = bits >= 0 ? bits : (bits ^ 0x7fffffffffffffffLL);
+    atomicMax((long long *)&out[1], bits);
+    atomicMax((unsigned long long *)&out[2], (unsigned long long)__double_as_longlong(ba));
+  }
+}
+
+/* --------------------------------------------------------------------------
+ * point-function kernels for the batch entry points
+ * ------------------------------------------------------------------------ */
+__global__ void kb_prim_to_cons(long n, const double *P, double *U, double gamma, int *err)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[8], u[8];
+  for (int q = 0; q < 8; ++q) p[q] = P[8 * i + q];
+  err[i] = prim_to_cons(p, u, gamma - 1.0);
+  for (int q = 0; q < 8; ++q) U[8 * i + q] = u[q];
+}
+
+__global__ void kb_cons_to_prim(long n, const double *U, double *P, double gamma, double pfloor,
+                                int *err, int *iters)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[8], u[8];
+  for (int q = 0; q < 8; ++q) { p[q] = P[8 * i + q]; u[q] = U[8 * i + q]; }
+  int it = 0, which = 0;
+  err[i] = cons_to_prim(u, p, gamma, pfloor, true, it, which);
+  iters[i] = it;
+  for (int q = 0; q < 8; ++q) P[8 * i + q] = p[q];
+}
+
+template <int DIM>
+__global__ void kb_flux_eigen(long n, const double *P, double *F, double *ap, double *am, double gamma)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[8], u[8], f[8], a, b;
+  for (int q = 0; q < 8; ++q) p[q] = P[8 * i + q];
+  prim_to_cons(p, u, gamma - 1.0);
+  flux_and_eigenvalues<double, DIM>(u, p, f, a, b, gamma);
+  for (int q = 0; q < 8; ++q) F[8 * i + q] = f[q];
+  ap[i] = a;
+  am[i] = b;
+}
+
+__global__ void kb_reconstruct(long n, const double *V, double *out, int op, double theta, int IS,
+                               double sz10A)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *v = V + 6 * i + 2;  // v[-2..3]
+  double r = 0.0;
+  switch (op) {
+  case 0: r = plm<double>(v[-1], v[0], v[1], -1.0, theta); break;  // PLM_C2L
+  case 1: r = plm<double>(v[-1], v[0], v[1], +1.0, theta); break;  // PLM_C2R
+  case 4: r = weno5<double, true>(v[-2], v[-1], v[0], v[1], v[2], IS, sz10A); break;
+  case 5: r = weno5<double, false>(v[-2], v[-1], v[0], v[1], v[2], IS, sz10A); break;
+  default: r = 0.0;
+  }
+  out[i] = r;
+}
+
+template <typename R, int DIM>
+__global__ void kb_riemann(long n, int solver, const double *PL, const double *PR, double *F,
+                           double gamma, double pfloor, int *err, double *lamo)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R pl[8], pr[8], f[8];
+  for (int q = 0; q < 8; ++q) { pl[q] = (R)PL[8 * i + q]; pr[q] = (R)PR[8 * i + q]; f[q] = R(0); }
+  R lam = R(0);
+  FluxCounters fc = {0, 0, 0, 0};
+  err[i] = riemann_dispatch<R, DIM>(solver, pl, pr, f, (R)gamma, pfloor, lam, fc);
+  for (int q = 0; q < 8; ++q) F[8 * i + q] = (double)f[q];
+  lamo[i] = (double)lam;
+}
+
+inline unsigned nblocks(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+/* ============================================================================
+ * context
+ * ========================================================================== */
+struct maragpu_ctx {
+  maragpu_config cfg;
+  GridDesc g;
+  hipStream_t stream = nullptr;
+  double *Pc = nullptr, *Pw = nullptr, *U0 = nullptr, *U = nullptr, *A = nullptr;
+  void *F[3] = {nullptr, nullptr, nullptr};
+  DevStats *dstats = nullptr;
+  DevStats *hstats = nullptr;       // pinned
+  double *ddiv = nullptr;
+  double max_lambda = 0.0;
+  bool have_state = false;
+  // per-kernel timing of the last advance
+  std::vector<hipEvent_t> ev;
+  double flux_ms = 0.0, c2p_ms = 0.0, upd_ms = 0.0;
+  int flux_n = 0, c2p_n = 0, upd_n = 0;
+  // learned c2p weights (device), see mlp.hip
+  void *mlp = nullptr;
+};
+
+namespace {
+
+int fail_free(maragpu_ctx *c)
+{
+  maragpu_destroy(c);
+  return MARAGPU_E_NOMEM;
+}
+
+GridDesc make_grid(const maragpu_config &c)
+{
+  // SimpleCartesianDomain (Mara/simple-cart.cpp:8-72) and prepare_integration (Mara/hydro.cpp:80-110)
+  GridDesc g;
+  double gdx[3];
+  double min_dx = c.x1[0] - c.x0[0];
+  for (int d = 0; d < 3; ++d) {
+    g.N[d] = c.N[d];
+    g.n[d] = c.N[d] + 2 * c.Ng;
+    gdx[d] = (c.x1[d] - c.x0[d]) / c.N[d];
+    if (gdx[d] < min_dx) min_dx = gdx[d];
+  }
+  g.Ng = c.Ng;
+  g.T = (long)g.n[0] * g.n[1] * g.n[2];
+  g.sz = 1;
+  g.sy = g.n[2];
+  g.sx = (long)g.n[1] * g.n[2];
+  double lx0[3], lx1[3];
+  for (int d = 0; d < 3; ++d) {
+    lx0[d] = c.x0[d] + gdx[d] * 0;
+    lx1[d] = c.x0[d] + gdx[d] * (0 + c.N[d]);
+  }
+  g.dx = (lx1[0] - lx0[0]) / (g.n[0] - 2 * c.Ng);
+  g.dy = (lx1[1] - lx0[1]) / (g.n[1] - 2 * c.Ng);
+  g.dz = (lx1[2] - lx0[2]) / (g.n[2] - 2 * c.Ng);
+  g.min_dx = min_dx;
+  return g;
+}
+
+template <typename R, typename FT, int DIM>
+void launch_flux(maragpu_ctx *c, const PhysParams &pp)
+{
+  const long S = DIM == 1 ? c->g.sx : DIM == 2 ? c->g.sy : c->g.sz;
+  const long n = c->g.T - 5 * S;
+  if (n <= 0) return;
+  FT *F = static_cast<FT *>(c->F[DIM - 1]);
+  switch (c->cfg.riemann) {
+  case MARAGPU_HLL:
+    k_flux<R, FT, DIM, RIEMANN_HLL><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats);
+    break;
+  case MARAGPU_HLLC:
+    k_flux<R, FT, DIM, RIEMANN_HLLC><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats);
+    break;
+  default:
+    k_flux<R, FT, DIM, RIEMANN_HLLD><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats);
+    break;
+  }
+}
+
+struct EventPair { int a, b; };
+
+}  // namespace
+
+extern "C" int maragpu_mlp_c2p_launch(maragpu_ctx *ctx, int first, int pass);
+
+extern "C" {
+
+const char *maragpu_strerror(int code)
+{
+  switch (code) {
+  case MARAGPU_OK: return "success";
+  case MARAGPU_E_INVALID: return "invalid argument or configuration";
+  case MARAGPU_E_HIP: return "HIP runtime error";
+  case MARAGPU_E_NOMEM: return "device allocation failed";
+  case MARAGPU_E_SIZE: return "buffer size does not match the domain";
+  case MARAGPU_E_NOMODEL: return "learned c2p requested but no network is loaded";
+  default: return code > 0 ? "zones failed to recover primitives (state not committed)" : "unknown error";
+  }
+}
+
+int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
+{
+  if (!cfg || !out) return MARAGPU_E_INVALID;
+  *out = nullptr;
+  for (int d = 0; d < 3; ++d)
+    if (cfg->N[d] < 1 || !(cfg->x1[d] > cfg->x0[d])) return MARAGPU_E_INVALID;
+  const int need_ng = cfg->recon == MARAGPU_WENO5 ? 3 : cfg->recon == MARAGPU_PLM ? 2 : 1;
+  if (cfg->Ng < need_ng || cfg->Ng > 8) return MARAGPU_E_INVALID;
+  if (cfg->rk < 1 || cfg->rk > 4 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->recon < 0 ||
+      cfg->recon > 2 || cfg->precision < 0 || cfg->precision > 1 || !(cfg->gamma > 1.0))
+    return MARAGPU_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MARAGPU_E_HIP;
+
+  maragpu_ctx *c = new (std::nothrow) maragpu_ctx;
+  if (!c) return MARAGPU_E_NOMEM;
+  c->cfg = *cfg;
+  c->g = make_grid(*cfg);
+  if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return MARAGPU_E_HIP; }
+  const size_t nd = 8 * (size_t)c->g.T;
+  const size_t fbytes = nd * (cfg->precision == MARAGPU_MIXED ? sizeof(float) : sizeof(double));
+  if (hipMalloc(&c->Pc, nd * sizeof(double)) != hipSuccess) return fail_free(c);
+  if (hipMalloc(&c->Pw, nd * sizeof(double)) != hipSuccess) return fail_free(c);
+  if (hipMalloc(&c->U0, nd * sizeof(double)) != hipSuccess) return fail_free(c);
+  if (hipMalloc(&c->U, nd * sizeof(double)) != hipSuccess) return fail_free(c);
+  if (cfg->rk == 4 && hipMalloc(&c->A, nd * sizeof(double)) != hipSuccess) return fail_free(c);
+  for (int d = 0; d < 3; ++d)
+    if (hipMalloc(&c->F[d], fbytes) != hipSuccess) return fail_free(c);
+  if (hipMalloc(&c->dstats, sizeof(DevStats)) != hipSuccess) return fail_free(c);
+  if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
+  if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
+  // fluxes outside the sweep ranges are never written; keep them defined
+  for (int d = 0; d < 3; ++d)
+    if (hipMemsetAsync(c->F[d], 0, fbytes, c->stream) != hipSuccess) return fail_free(c);
+  if (hipMemsetAsync(c->Pc, 0, nd * sizeof(double), c->stream) != hipSuccess) return fail_free(c);
+  c->ev.resize(64);
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) { e = nullptr; return fail_free(c); }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return fail_free(c);
+  *out = c;
+  return MARAGPU_OK;
+}
+
+extern void maragpu_mlp_free(void *);
+
+int maragpu_destroy(maragpu_ctx *c)
+{
+  if (!c) return MARAGPU_E_INVALID;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  hipFree(c->Pc); hipFree(c->Pw); hipFree(c->U0); hipFree(c->U); hipFree(c->A);
+  for (int d = 0; d < 3; ++d) hipFree(c->F[d]);
+  hipFree(c->dstats);
+  if (c->hstats) hipHostFree(c->hstats);
+  hipFree(c->ddiv);
+  for (auto &e : c->ev) if (e) hipEventDestroy(e);
+  if (c->mlp) maragpu_mlp_free(c->mlp);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return MARAGPU_OK;
+}
+
+int maragpu_upload_prim(maragpu_ctx *c, const double *P, size_t n)
+{
+  if (!c || !P) return MARAGPU_E_INVALID;
+  if (n != 8 * (size_t)c->g.T) return MARAGPU_E_SIZE;
+  HIP_CHECK(hipMemcpyAsync(c->U0, P, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_aos_to_soa<<<nblocks(c->g.T), kBlock, 0, c->stream>>>(c->U0, c->Pc, c->g.T);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->have_state = true;
+  return MARAGPU_OK;
+}
+
+int maragpu_download_prim(maragpu_ctx *c, double *P, size_t n)
+{
+  if (!c || !P) return MARAGPU_E_INVALID;
+  if (n != 8 * (size_t)c->g.T) return MARAGPU_E_SIZE;
+  k_soa_to_aos<<<nblocks(c->g.T), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->g.T);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(P, c->U0, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MARAGPU_OK;
+}
+
+int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
+{
+  if (!c) return MARAGPU_E_INVALID;
+  if (!c->have_state) return MARAGPU_E_INVALID;
+  if (c->cfg.c2p == MARAGPU_C2P_LEARNED && !c->mlp) return MARAGPU_E_NOMODEL;
+  const GridDesc &g = c->g;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  PhysParams pp;
+  pp.gamma = c->cfg.gamma;
+  pp.pfloor = c->cfg.pfloor;
+  pp.theta = c->cfg.theta;
+  pp.IS = c->cfg.IS;
+  pp.sz10A = c->cfg.sz10A;
+  const bool mixed = c->cfg.precision == MARAGPU_MIXED;
+  int evi = 0;
+  std::vector<EventPair> ef, ec, eu;
+  auto mark = [&](std::vector<EventPair> &v) {
+    EventPair p = { evi, evi + 1 };
+    evi += 2;
+    v.push_back(p);
+    return p;
+  };
+
+  HIP_CHECK(hipMemsetAsync(c->dstats, 0, sizeof(DevStats), c->stream));
+  HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));
+  const int ev_start = evi++;
+  k_prim_to_cons<<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, g, c->cfg.gamma);
+
+  auto c2p = [&](int first, int pass) -> int {
+    EventPair p = mark(ec);
+    HIP_CHECK(hipEventRecord(c->ev[p.a], c->stream));
+    if (c->cfg.c2p == MARAGPU_C2P_LEARNED) {
+      const int rc = maragpu_mlp_c2p_launch(c, first, pass);
+      if (rc) return rc;
+    }
+    else {
+      k_cons_to_prim<<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+                                                            first, c->cfg.ereset, c->cfg.preset,
+                                                            c->cfg.dreset, c->dstats, pass);
+    }
+    HIP_CHECK(hipEventRecord(c->ev[p.b], c->stream));
+    return 0;
+  };
+  auto sweeps = [&]() {
+    for (int d = 1; d <= 3; ++d) {
+      EventPair p = mark(ef);
+      hipEventRecord(c->ev[p.a], c->stream);
+      if (mixed) {
+        if (d == 1) launch_flux<float, float, 1>(c, pp);
+        if (d == 2) launch_flux<float, float, 2>(c, pp);
+        if (d == 3) launch_flux<float, float, 3>(c, pp);
+      }
+      else {
+        if (d == 1) launch_flux<double, double, 1>(c, pp);
+        if (d == 2) launch_flux<double, double, 2>(c, pp);
+        if (d == 3) launch_flux<double, double, 3>(c, pp);
+      }
+      hipEventRecord(c->ev[p.b], c->stream);
+    }
+  };
+  auto update = [&](int kind) {
+    EventPair p = mark(eu);
+    hipEventRecord(c->ev[p.a], c->stream);
+    if (mixed)
+      k_update<float><<<nblocks(ni), kBlock, 0, c->stream>>>((const float *)c->F[0], (const float *)c->F[1],
+                                                              (const float *)c->F[2], c->U0, c->U, c->A, g, dt, kind);
+    else
+      k_update<double><<<nblocks(ni), kBlock, 0, c->stream>>>((const double *)c->F[0], (const double *)c->F[1],
+                                                               (const double *)c->F[2], c->U0, c->U, c->A, g, dt, kind);
+    hipEventRecord(c->ev[p.b], c->stream);
+  };
+
+  std::vector<int> kinds;
+  switch (c->cfg.rk) {
+  case 1: kinds = { ST_EULER }; break;
+  case 2: kinds = { ST_EULER, ST_RK2_FINAL }; break;
+  case 4: kinds = { ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4 }; break;
+  default: kinds = { ST_EULER, ST_RK3_2, ST_RK3_3 }; break;
+  }
+  int pass = 0;
+  for (size_t s = 0; s < kinds.size(); ++s) {
+    int rc = c2p(s == 0, pass++);
+    if (rc) return rc;
+    sweeps();
+    update(kinds[s]);
+  }
+  int rc = c2p(0, pass++);
+  if (rc) return rc;
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));
+  const int ev_end = evi++;
+  HIP_CHECK(hipMemcpyAsync(c->hstats, c->dstats, sizeof(DevStats), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+
+  const DevStats &h = *c->hstats;
+  int errors = 0;
+  for (int p = 0; p < pass; ++p) {
+    if (h.stage_failed[p]) { errors = (int)h.stage_failed[p]; break; }
+  }
+  double lam;
+  unsigned long long bits = h.lam_bits;
+  std::memcpy(&lam, &bits, sizeof(lam));
+  c->max_lambda = lam;
+  if (errors == 0) std::swap(c->Pc, c->Pw);
+
+  auto sum_ms = [&](const std::vector<EventPair> &v) {
+    double t = 0.0;
+    for (auto &p : v) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, c->ev[p.a], c->ev[p.b]);
+      t += ms;
+    }
+    return t;
+  };
+  c->flux_ms = sum_ms(ef); c->flux_n = (int)ef.size();
+  c->c2p_ms = sum_ms(ec); c->c2p_n = (int)ec.size();
+  c->upd_ms = sum_ms(eu); c->upd_n = (int)eu.size();
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->max_lambda = lam;
+    stats->hlld_failure = (int64_t)h.hlld_failure;
+    stats->hlld_success = (int64_t)h.hlld_success;
+    stats->riemann_retries = (int64_t)h.retries;
+    stats->riemann_failures = (int64_t)h.failures;
+    stats->c2p_zones = (int64_t)h.c2p_zones;
+    stats->c2p_iterations = (int64_t)h.c2p_iters;
+    stats->c2p_fallbacks = (int64_t)h.c2p_fallbacks;
+    stats->c2p_failed = errors;
+    stats->hlld_c2p_iterations = (int64_t)h.hlld_c2p_iters;
+    stats->hlld_newton_iterations = (int64_t)h.hlld_newton_iters;
+    stats->mlp_guesses = (int64_t)h.mlp_guesses;
+    stats->mlp_fallbacks = (int64_t)h.mlp_fallbacks;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev[ev_start], c->ev[ev_end]);
+    stats->device_ms = ms;
+  }
+  return errors;
+}
+
+double maragpu_get_timestep(maragpu_ctx *c, double cfl)
+{
+  if (!c) return std::numeric_limits<double>::quiet_NaN();
+  return cfl * c->g.min_dx / c->max_lambda;
+}
+
+int maragpu_measure_divB(maragpu_ctx *c, double *mean, double *max, double *absmax)
+{
+  if (!c) return MARAGPU_E_INVALID;
+  const GridDesc &g = c->g;
+  const long cnt = (long)(g.n[0] - 2 * g.Ng - 1) * (g.n[1] - 2 * g.Ng - 1) * (g.n[2] - 2 * g.Ng - 1);
+  double init[3] = { 0.0, 0.0, 0.0 };
+  {
+    // signed-max slot starts at the integer image of -inf
+    long long ninf = __builtin_bit_cast(long long, -std::numeric_limits<double>::infinity());
+    ninf = ninf ^ 0x7fffffffffffffffLL;
+    std::memcpy(&init[1], &ninf, sizeof(double));
+  }
+  HIP_CHECK(hipMemcpyAsync(c->ddiv, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+  if (cnt > 0) k_divB<<<nblocks(cnt), kBlock, 0, c->stream>>>(c->Pc, g, c->ddiv);
+  double r[3];
+  HIP_CHECK(hipMemcpyAsync(r, c->ddiv, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  long long bits;
+  std::memcpy(&bits, &r[1], sizeof(bits));
+  bits = bits >= 0 ? bits : (bits ^ 0x7fffffffffffffffLL);
+  double smax;
+  std::memcpy(&smax, &bits, sizeof(smax));
+  // zones outside the stencil's loop hold 0 in the reference's div array
+  if (g.T > cnt && smax < 0.0) smax = 0.0;
+  const double glb = (double)g.N[0] * g.N[1] * g.N[2];
+  if (mean) *mean = r[0] / (double)g.T / glb;
+  if (max) *max = smax;
+  if (absmax) *absmax = r[2] / g.min_dx;
+  return MARAGPU_OK;
+}
+
+int maragpu_kernel_times(maragpu_ctx *c, double *flux_ms, int *flux_n, double *c2p_ms, int *c2p_n,
+                         double *upd_ms, int *upd_n)
+{
+  if (!c) return MARAGPU_E_INVALID;
+  if (flux_ms) *flux_ms = c->flux_ms;
+  if (flux_n) *flux_n = c->flux_n;
+  if (c2p_ms) *c2p_ms = c->c2p_ms;
+  if (c2p_n) *c2p_n = c->c2p_n;
+  if (upd_ms) *upd_ms = c->upd_ms;
+  if (upd_n) *upd_n = c->upd_n;
+  return MARAGPU_OK;
+}
+
+}  // extern "C"
+
+/* ---------------------------------------------------------------------------
+ * batch entry points
+ * ------------------------------------------------------------------------- */
+namespace {
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) hipFree(p); }
+  template <typename T> T *get() { return static_cast<T *>(p); }
+};
+int dalloc(DevBuf &b, size_t bytes) { return hipMalloc(&b.p, bytes ? bytes : 1) == hipSuccess ? 0 : MARAGPU_E_NOMEM; }
+}  // namespace
+
+extern "C" {
+
+int maragpu_batch_prim_to_cons(size_t n, const double *P, double *U, double gamma, int *err)
+{
+  if (!P || !U || !err) return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  DevBuf dP, dU, dE;
+  if (dalloc(dP, n * 64) || dalloc(dU, n * 64) || dalloc(dE, n * 4)) return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dP.p, P, n * 64, hipMemcpyHostToDevice));
+  kb_prim_to_cons<<<nblocks(n), kBlock>>>((long)n, dP.get<double>(), dU.get<double>(), gamma, dE.get<int>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(U, dU.p, n * 64, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(err, dE.p, n * 4, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+int maragpu_batch_cons_to_prim(size_t n, const double *U, double *P, double gamma, double pfloor,
+                               int *err, int *iters)
+{
+  if (!P || !U || !err || !iters) return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  DevBuf dP, dU, dE, dI;
+  if (dalloc(dP, n * 64) || dalloc(dU, n * 64) || dalloc(dE, n * 4) || dalloc(dI, n * 4)) return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dP.p, P, n * 64, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(dU.p, U, n * 64, hipMemcpyHostToDevice));
+  kb_cons_to_prim<<<nblocks(n), kBlock>>>((long)n, dU.get<double>(), dP.get<double>(), gamma, pfloor,
+                                          dE.get<int>(), dI.get<int>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(P, dP.p, n * 64, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(err, dE.p, n * 4, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(iters, dI.p, n * 4, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+int maragpu_batch_flux_eigen(size_t n, const double *P, double *F, double *ap, double *am, int dim,
+                             double gamma)
+{
+  if (!P || !F || !ap || !am || dim < 1 || dim > 3) return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  DevBuf dP, dF, dA, dB;
+  if (dalloc(dP, n * 64) || dalloc(dF, n * 64) || dalloc(dA, n * 8) || dalloc(dB, n * 8)) return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dP.p, P, n * 64, hipMemcpyHostToDevice));
+  auto *p = dP.get<double>(); auto *f = dF.get<double>(); auto *a = dA.get<double>(); auto *b = dB.get<double>();
+  if (dim == 1) kb_flux_eigen<1><<<nblocks(n), kBlock>>>((long)n, p, f, a, b, gamma);
+  if (dim == 2) kb_flux_eigen<2><<<nblocks(n), kBlock>>>((long)n, p, f, a, b, gamma);
+  if (dim == 3) kb_flux_eigen<3><<<nblocks(n), kBlock>>>((long)n, p, f, a, b, gamma);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(F, dF.p, n * 64, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(ap, dA.p, n * 8, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(am, dB.p, n * 8, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+int maragpu_batch_reconstruct(size_t n, const double *V, double *out, int op, double theta, int IS,
+                              double sz10A)
+{
+  if (!V || !out || !(op == 0 || op == 1 || op == 4 || op == 5)) return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  DevBuf dV, dO;
+  if (dalloc(dV, n * 48) || dalloc(dO, n * 8)) return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dV.p, V, n * 48, hipMemcpyHostToDevice));
+  kb_reconstruct<<<nblocks(n), kBlock>>>((long)n, dV.get<double>(), dO.get<double>(), op, theta, IS, sz10A);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(out, dO.p, n * 8, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+int maragpu_batch_riemann(size_t n, int solver, int precision, const double *PL, const double *PR,
+                          double *F, int dim, double gamma, double pfloor, int *err, double *lam)
+{
+  if (!PL || !PR || !F || !err || !lam || dim < 1 || dim > 3 || solver < 0 || solver > 2)
+    return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  DevBuf dL, dR, dF, dE, dM;
+  if (dalloc(dL, n * 64) || dalloc(dR, n * 64) || dalloc(dF, n * 64) || dalloc(dE, n * 4) ||
+      dalloc(dM, n * 8))
+    return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dL.p, PL, n * 64, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(dR.p, PR, n * 64, hipMemcpyHostToDevice));
+  auto *l = dL.get<double>(); auto *r = dR.get<double>(); auto *f = dF.get<double>();
+  auto *e = dE.get<int>(); auto *m = dM.get<double>();
+  const unsigned nb = nblocks(n);
+  if (precision == MARAGPU_MIXED) {
+    if (dim == 1) kb_riemann<float, 1><<<nb, kBlock>>>((long)n, solver, l, r, f, gamma, pfloor, e, m);
+    if (dim == 2) kb_riemann<float, 2><<<nb, kBlock>>>((long)n, solver, l, r, f, gamma, pfloor, e, m);
+    if (dim == 3) kb_riemann<float, 3><<<nb, kBlock>>>((long)n, solver, l, r, f, gamma, pfloor, e, m);
+  }
+  else {
+    if (dim == 1) kb_riemann<double, 1><<<nb, kBlock>>>((long)n, solver, l, r, f, gamma, pfloor, e, m);
+    if (dim == 2) kb_riemann<double, 2><<<nb, kBlock>>>((long)n, solver, l, r, f, gamma, pfloor, e, m);
+    if (dim == 3) kb_riemann<double, 3><<<nb, kBlock>>>((long)n, solver, l, r, f, gamma, pfloor, e, m);
+  }
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(F, dF.p, n * 64, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(err, dE.p, n * 4, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(lam, dM.p, n * 8, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+}  // extern "C"

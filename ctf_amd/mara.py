@@ -1,0 +1,112 @@
+"""``Mara`` — the Lua-facing verbs of Mara (luaopen_Mara, Mara/mara.cpp:137-268) over
+the device path. A Lua driver such as ctf-core/run-mara.lua:113-148 reads the same here::
+
+    from ctf_amd.mara import Mara
+    Mara.start()
+    Mara.set_fluid('rmhd')
+    Mara.set_eos('gamma-law', 4/3)
+    Mara.set_advance('rk2')
+    Mara.set_godunov('plm-split')
+    Mara.set_boundary('periodic')
+    Mara.set_riemann('hlld')
+    Mara.config_solver({'theta': 2.0, 'IS': 'js96', 'sz10A': 100.0}, False)
+    Mara.set_domain({0,0,0}, {1,1,1}, {64,64,64}, 8, 3)
+    kzps, nerr = Mara.advance(P, dt)          # P: numpy AoS buffer, updated in place
+    dt = Mara.get_timestep(0.4)
+
+Misuse raises ``MaraError`` where the Lua binding calls luaL_error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native
+
+
+class MaraError(RuntimeError):
+    pass
+
+
+def _s(x: str) -> bytes:
+    return x.encode()
+
+
+def _chk(rc):
+    if rc < 0:
+        raise MaraError(native.lib().mara_last_error().decode())
+    return rc
+
+
+class _Measure:
+    def mean_max_divB(self):
+        m, x = C.c_double(), C.c_double()
+        _chk(native.lib().mara_measure_mean_max_divB(C.byref(m), C.byref(x)))
+        return m.value, x.value
+
+
+class _Mara:
+    measure = _Measure()
+
+    def start(self):
+        _chk(native.lib().mara_start())
+
+    def close(self):
+        _chk(native.lib().mara_close())
+
+    def set_fluid(self, key):
+        _chk(native.lib().mara_set_fluid(_s(key)))
+
+    def set_eos(self, key, gamma=1.4):
+        _chk(native.lib().mara_set_eos(_s(key), float(gamma)))
+
+    def set_boundary(self, key):
+        _chk(native.lib().mara_set_boundary(_s(key)))
+
+    def set_riemann(self, key):
+        _chk(native.lib().mara_set_riemann(_s(key)))
+
+    def set_godunov(self, key):
+        _chk(native.lib().mara_set_godunov(_s(key)))
+
+    def set_advance(self, key):
+        _chk(native.lib().mara_set_advance(_s(key)))
+
+    def config_solver(self, table: dict, quiet: bool = False):
+        L = native.lib()
+        for k, v in table.items():
+            if isinstance(v, bool):
+                _chk(L.mara_config_solver_bool(_s(k), int(v)))
+            elif isinstance(v, (int, float)):
+                _chk(L.mara_config_solver_num(_s(k), float(v)))
+            elif isinstance(v, str):
+                _chk(L.mara_config_solver_str(_s(k), _s(v)))
+            if not quiet:
+                print(f"[Mara] setting {k}={v}")
+
+    def set_domain(self, x0, x1, N, Nq, Ng, comm=None):
+        x0 = np.ascontiguousarray(list(x0), dtype=np.float64)
+        x1 = np.ascontiguousarray(list(x1), dtype=np.float64)
+        n = np.ascontiguousarray(list(N), dtype=np.int32)
+        _chk(native.lib().mara_set_domain(native.dptr(x0), native.dptr(x1), native.iptr(n),
+                                          len(n), int(Nq), int(Ng)))
+
+    def advance(self, P: np.ndarray, dt: float):
+        """Advances the AoS primitive buffer P in place (unless zones fail).
+        Returns (kz/s, number of failed zones) as the Lua binding does."""
+        if P.dtype != np.float64 or not P.flags.c_contiguous:
+            raise MaraError("[Mara] primitive buffer must be contiguous float64")
+        kz, ne = C.c_double(), C.c_int()
+        _chk(native.lib().mara_advance(native.dptr(P), P.size, float(dt), C.byref(kz), C.byref(ne)))
+        return kz.value, ne.value
+
+    def get_timestep(self, cfl):
+        return native.lib().mara_get_timestep(float(cfl))
+
+    def set_primitive(self, P: np.ndarray):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        _chk(native.lib().mara_set_primitive(native.dptr(P), P.size))
+
+
+Mara = _Mara()

@@ -1,0 +1,75 @@
+/* mara_host — Mara's Lua-facing operator API over the maragpu device path, C ABI.
+ *
+ * The verbs keep the names, argument meaning and error behaviour of the Lua table that
+ * luaopen_Mara registers (Mara/mara.cpp:137-268); where the Lua binding raises
+ * luaL_error these return a negative code and mara_last_error() holds the message.
+ * One application per process, as in Mara (the `Mara` global, Mara/mara.cpp).
+ *
+ *   mara_start / mara_close          luaC_Mara_start / close         (Mara/mara.cpp:270-295)
+ *   mara_set_fluid                   luaC_set_fluid                  (:813-837)   'rmhd'
+ *   mara_set_eos                     luaC_set_eos                    (:840-867)   'gamma-law', Gamma
+ *   mara_set_boundary                luaC_set_boundary               (:887-932)   'periodic'
+ *   mara_set_riemann                 luaC_set_riemann                (:934-958)   'hll' 'hllc' 'hlld'
+ *   mara_set_godunov                 luaC_set_godunov                (:960-984)   'plm-split'
+ *   mara_set_advance                 luaC_set_advance                (:1140-1167) 'single' 'rk2' 'rk3' 'rk4'
+ *   mara_config_solver_{str,num,bool} one key of luaC_config_solver  (:996-1138); extra keys
+ *                                    'precision' = 'fp64' | 'mixed' and 'c2p' = 'plain' | 'learned'
+ *   mara_set_domain                  luaC_set_domain                 (:726-755)
+ *   mara_advance                     luaC_Mara_advance               (:297-341) -> kz/s, errors
+ *   mara_get_timestep                luaC_Mara_get_timestep          (:374-381)
+ *   mara_set_primitive               luaC_set_primitive              (:1284-1295)
+ *   mara_measure_mean_max_divB       luaC_mean_max_divB              (:2264-2324)
+ *   mara_init_turbulence             (synthetic data; the role of an init_prim callback, :531-632)
+ */
+#ifndef MARA_HOST_H
+#define MARA_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int mara_start(void);
+int mara_close(void);
+int mara_set_fluid(const char *key);
+int mara_set_eos(const char *key, double gamma);
+int mara_set_boundary(const char *key);
+int mara_set_riemann(const char *key);
+int mara_set_godunov(const char *key);
+int mara_set_advance(const char *key);
+int mara_config_solver_str(const char *key, const char *value);
+int mara_config_solver_num(const char *key, double value);
+int mara_config_solver_bool(const char *key, int value);
+int mara_set_domain(const double *x0, const double *x1, const int *N, int Nd, int Nq, int Ng);
+/* P: AoS primitives, n doubles; on return kzps = 1e-3 zones (ghosts included) per wall
+ * second and nerr = failed zones (P untouched when nerr > 0) */
+int mara_advance(double *P, size_t n, double dt, double *kzps, int *nerr);
+double mara_get_timestep(double cfl);
+int mara_set_primitive(const double *P, size_t n);
+int mara_measure_mean_max_divB(double *mean, double *max);
+const char *mara_last_error(void);
+/* the device context behind the application (created on first use), for callers that
+ * want to keep the state on the device between steps */
+struct maragpu_ctx *mara_device_context(void);
+
+typedef struct mara_ic_params {
+  int N[3];
+  int Ng;
+  double rho, pre;
+  double B0[3];
+  double v_rms, v_kmin, v_kmax;
+  int four_velocity;       /* v_rms is the rms of u = W v */
+  double dB_rms, A_kmin, A_kmax;
+  uint64_t seed;
+} mara_ic_params;
+
+/* fills an AoS primitive array of n = (N+2Ng)^3 x 8 doubles; 0 on success */
+int mara_init_turbulence(const mara_ic_params *ic, double *P, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARA_HOST_H */

@@ -1,0 +1,123 @@
+/* maragpu — MI355X (gfx950) relativistic-MHD update, C ABI.
+ *
+ * One context owns one device-resident grid and one HIP stream. Nothing in this header
+ * uses HIP or torch types; all pointers are host pointers unless stated otherwise.
+ * Every entry point returns instead of throwing. Return codes: 0 success, > 0 a count
+ * (failed zones, for maragpu_advance), < 0 an error (maragpu_strerror names it).
+ *
+ * Reference interfaces replaced (jzrake/ctf, Mara):
+ *   maragpu_create / maragpu_destroy   MaraApplication set-up: Mara.set_domain, set_fluid('rmhd'),
+ *                                      set_eos('gamma-law', G), set_godunov('plm-split'),
+ *                                      set_riemann, set_advance, config_solver
+ *                                      (Mara/mara.cpp:726-755, 813-867, 934-1167)
+ *   maragpu_upload_prim / download     the memcpy in/out of luaC_Mara_advance (Mara/mara.cpp:317,333),
+ *                                      host layout = Mara's AoS [i][j][k][q] (Mara/valman.hpp:25-36)
+ *   maragpu_advance                    luaC_Mara_advance body (Mara/mara.cpp:297-341):
+ *                                      GodunovOperator::PrimToCons, RungeKuttaIntegration::AdvanceState,
+ *                                      ConsToPrim; state is not committed when zones fail
+ *   maragpu_get_timestep               luaC_Mara_get_timestep (Mara/mara.cpp:374-381)
+ *   maragpu_measure_divB               luaC_mean_max_divB (Mara/mara.cpp:2264-2324)
+ *   maragpu_batch_prim_to_cons         AdiabaticIdealRmhd::PrimToCons (Mara/rmhd.cpp:133-163)
+ *   maragpu_batch_cons_to_prim         AdiabaticIdealRmhd::ConsToPrim cascade (Mara/rmhd.cpp:59-132)
+ *                                      over the rmhd-c2p.h C API (Mara/rmhd-c2p.h:22-35)
+ *   maragpu_batch_flux_eigen           AdiabaticIdealRmhd::FluxAndEigenvalues (Mara/rmhd.cpp:164-265)
+ *   maragpu_batch_reconstruct          reconstruct(v, op) of weno.h (Mara/weno.h:17-20, weno.c:58-71)
+ *   maragpu_batch_riemann              RiemannSolver::IntercellFlux (Mara/hydro.hpp:217-218) of
+ *                                      HllRiemannSolver, HllcRmhdRiemannSolver, HlldRmhdRiemannSolver
+ */
+#ifndef MARAGPU_H
+#define MARAGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MARAGPU_PCM = 0, MARAGPU_PLM = 1, MARAGPU_WENO5 = 2 };            /* config_solver extrap */
+enum { MARAGPU_JS96 = 0, MARAGPU_B08 = 1, MARAGPU_SZ10 = 2 };            /* config_solver IS */
+enum { MARAGPU_HLL = 0, MARAGPU_HLLC = 1, MARAGPU_HLLD = 2 };            /* set_riemann */
+enum { MARAGPU_FP64 = 0, MARAGPU_MIXED = 1 };                            /* precision */
+enum { MARAGPU_C2P_PLAIN = 0, MARAGPU_C2P_LEARNED = 1 };                 /* c2p initial guess */
+
+enum {
+  MARAGPU_OK = 0,
+  MARAGPU_E_INVALID = -1,   /* bad argument or configuration */
+  MARAGPU_E_HIP = -2,       /* HIP runtime error */
+  MARAGPU_E_NOMEM = -3,     /* device allocation failed */
+  MARAGPU_E_SIZE = -4,      /* buffer size does not match the domain */
+  MARAGPU_E_NOMODEL = -5    /* learned c2p requested without loaded weights */
+};
+
+typedef struct maragpu_config {
+  int N[3];          /* interior zones per axis */
+  int Ng;            /* ghost zones per side (3 for WENO5) */
+  double x0[3], x1[3];
+  double gamma;      /* gamma-law index */
+  int recon;         /* MARAGPU_PCM / PLM / WENO5 */
+  double theta;      /* PLM theta (2 = MC) */
+  int IS;            /* WENO smoothness indicator */
+  double sz10A;
+  int riemann;       /* MARAGPU_HLL / HLLC / HLLD */
+  int rk;            /* 1 single, 2 rk2, 3 rk3, 4 rk4 */
+  int precision;     /* MARAGPU_FP64 or MARAGPU_MIXED */
+  int c2p;           /* MARAGPU_C2P_PLAIN or MARAGPU_C2P_LEARNED */
+  double pfloor;     /* pressure floor, disabled if <= 0 */
+  int ereset;        /* emergency reset of failed zones */
+  double preset, dreset;
+} maragpu_config;
+
+typedef struct maragpu_stats {
+  double max_lambda;            /* RiemannSolver::MaxLambda over the step */
+  int64_t hlld_success;         /* HLLD solves that produced the flux */
+  int64_t hlld_failure;         /* HLLD solves that fell back to HLLC */
+  int64_t riemann_retries;      /* interfaces retried with the piecewise-constant states */
+  int64_t riemann_failures;     /* interfaces where both attempts failed (flux set to 0) */
+  int64_t c2p_zones;            /* per-zone c2p solves (interior zones x (stages + 1)) */
+  int64_t c2p_iterations;       /* Newton iterations of those solves */
+  int64_t c2p_fallbacks;        /* per-zone solves not settled by their first attempt */
+  int64_t c2p_failed;           /* failed zones of the failing stage, ghost images included */
+  int64_t hlld_c2p_iterations;  /* Newton iterations of the c2p nested in HLLD */
+  int64_t hlld_newton_iterations; /* p* Newton iterations in HLLD */
+  int64_t mlp_guesses;          /* learned c2p: zones started from the network's guess */
+  int64_t mlp_fallbacks;        /* learned c2p: zones that fell back to the plain cascade */
+  double device_ms;             /* device time of the step */
+} maragpu_stats;
+
+typedef struct maragpu_ctx maragpu_ctx;
+
+int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out);
+int maragpu_destroy(maragpu_ctx *ctx);
+/* n = number of doubles = zones (ghosts included) x 8 */
+int maragpu_upload_prim(maragpu_ctx *ctx, const double *P_aos, size_t n);
+int maragpu_download_prim(maragpu_ctx *ctx, double *P_aos, size_t n);
+int maragpu_advance(maragpu_ctx *ctx, double dt, maragpu_stats *stats);
+/* CFL * min dx / MaxLambda of the last step: +inf before any solve, as in Mara */
+double maragpu_get_timestep(maragpu_ctx *ctx, double cfl);
+/* mean and signed max in the reference's convention; absmax = max |corner div B| / dx */
+int maragpu_measure_divB(maragpu_ctx *ctx, double *mean, double *max, double *absmax);
+/* device time of the dominant kernel over the last advance (ms, per launch) and launches */
+int maragpu_kernel_times(maragpu_ctx *ctx, double *flux_ms, int *flux_launches, double *c2p_ms,
+                         int *c2p_launches, double *update_ms, int *update_launches);
+const char *maragpu_strerror(int code);
+
+/* Point functions over n states on the device (host arrays in and out). */
+int maragpu_batch_prim_to_cons(size_t n, const double *P, double *U, double gamma, int *err);
+/* P holds the previous primitive states (the cascade's second guess) on entry */
+int maragpu_batch_cons_to_prim(size_t n, const double *U, double *P, double gamma, double pfloor,
+                               int *err, int *iters);
+int maragpu_batch_flux_eigen(size_t n, const double *P, double *F, double *ap, double *am, int dim,
+                             double gamma);
+/* V holds n stencils of 6 values v[-2..3]; op as weno.h ReconstructOperation (PLM_C2L=0,
+ * PLM_C2R=1, WENO5_FV_C2R=4, WENO5_FV_C2L=5) applied at v[0] */
+int maragpu_batch_reconstruct(size_t n, const double *V, double *out, int op, double theta, int IS,
+                              double sz10A);
+int maragpu_batch_riemann(size_t n, int solver, int precision, const double *PL, const double *PR,
+                          double *F, int dim, double gamma, double pfloor, int *err, double *lam);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MARAGPU_H */

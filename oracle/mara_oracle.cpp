@@ -1,0 +1,601 @@
+/* ORACLE — test infrastructure only (the checker, never the product).
+ *
+ * Grid-level CPU restatement of Mara's plain-method time step, following
+ *   Mara/mara.cpp:297-341        luaC_Mara_advance (copy-in, P->U, RK, U->P, commit)
+ *   Mara/hydro.cpp:80-202        prepare_integration, PrimToCons, ConsToPrim (+ emergency reset)
+ *   Mara/plm-split.cpp:11-120    dUdt, intercell_flux_sweep, drive_sweeps_3d
+ *   Mara/rmhd.cpp:293-337        ConstrainedTransport3d
+ *   Mara/runge-kutta.hpp:7-55    single / rk2 / rk3 / rk4
+ *   Mara/simple-cart.cpp:81-170  periodic Synchronize
+ *   Mara/mara.cpp:374-381        get_timestep
+ *   Mara/mara.cpp:2264-2324      measure.mean_max_divB
+ * Arrays are the reference's own layout: AoS [i][j][k][q], q fastest, ghosts included.
+ * Loops over zones run under OpenMP; each zone's arithmetic is the reference's, and
+ * the only reductions are a max and integer sums, so results do not depend on the
+ * thread count.
+ *
+ * Exposed through a C ABI (liboracle.so) for the Python tests and bench's CPU
+ * baseline; built by oracle/Makefile.
+ */
+#include "mara_oracle.hpp"
+#include "counted.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace oracle {
+thread_local FlopTally g_tally;
+}
+
+using namespace oracle;
+
+extern "C" {
+
+typedef struct {
+  int N[3];
+  int Ng;
+  double x0[3], x1[3];
+  double gamma;
+  int recon;      /* 0 pcm, 1 plm, 2 weno5 */
+  double theta;
+  int IS;         /* 0 js96, 1 b08, 2 sz10 */
+  double sz10A;
+  int riemann;    /* 0 hll, 1 hllc, 2 hlld */
+  int rk;         /* 1 single, 2 rk2, 3 rk3, 4 rk4 */
+  double pfloor;
+  int ereset;
+  double preset, dreset;
+} oracle_config;
+
+typedef struct {
+  double max_lambda;
+  double hlld_success;
+  double hlld_failure;
+  double c2p_iterations;
+  double c2p_solves;
+  double riemann_retries;  /* interfaces that fell back to the PCM states */
+  double riemann_failures; /* interfaces where both attempts failed */
+  double reserved;
+} oracle_stats;
+
+}  // extern "C"
+
+namespace {
+
+struct Grid {
+  oracle_config cfg;
+  int n[3];        // augmented shape
+  long T;          // total zones
+  long sx, sy, sz; // strides in doubles (stride[1..3] of prepare_integration)
+  double dx, dy, dz, min_dx;
+  PhysParams pp;
+  double max_lambda;
+  std::vector<double> prim_array;   // MaraApplication::PrimitiveArray
+  std::vector<int> failure_mask;    // MaraApplication::FailureMask
+  oracle_stats last;
+};
+
+inline long wrap(long i, int N, int Ng)
+{
+  // index of the interior image of augmented index i (periodic)
+  if (i < Ng) return i + N;
+  if (i >= N + Ng) return i - N;
+  return i;
+}
+
+void synchronize(const Grid &g, double *A)
+{
+  // Mara/simple-cart.cpp:81-170: all ghost zones end up holding their periodic image.
+  const int Ng = g.cfg.Ng;
+  const int *N = g.cfg.N;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < g.n[0]; ++i) {
+    for (long j = 0; j < g.n[1]; ++j) {
+      for (long k = 0; k < g.n[2]; ++k) {
+        const bool ghost = i < Ng || i >= N[0] + Ng || j < Ng || j >= N[1] + Ng ||
+                           k < Ng || k >= N[2] + Ng;
+        if (!ghost) continue;
+        const long src = ((wrap(i, N[0], Ng) * g.n[1] + wrap(j, N[1], Ng)) * g.n[2] +
+                          wrap(k, N[2], Ng)) * 8;
+        const long dst = ((i * g.n[1] + j) * g.n[2] + k) * 8;
+        std::memcpy(A + dst, A + src, 8 * sizeof(double));
+      }
+    }
+  }
+}
+
+void prepare(Grid &g)
+{
+  // Mara/simple-cart.cpp:8-72 and Mara/hydro.cpp:80-110
+  const oracle_config &c = g.cfg;
+  double gdx[3];
+  double min_dx = c.x1[0] - c.x0[0];
+  for (int d = 0; d < 3; ++d) {
+    gdx[d] = (c.x1[d] - c.x0[d]) / c.N[d];
+    if (gdx[d] < min_dx) min_dx = gdx[d];
+  }
+  g.min_dx = min_dx;
+  double lx0[3], lx1[3];
+  for (int d = 0; d < 3; ++d) {
+    lx0[d] = c.x0[d] + gdx[d] * 0;
+    lx1[d] = c.x0[d] + gdx[d] * (0 + c.N[d]);
+    g.n[d] = c.N[d] + 2 * c.Ng;
+  }
+  g.T = (long)g.n[0] * g.n[1] * g.n[2];
+  g.sz = 8;
+  g.sy = (long)g.n[2] * 8;
+  g.sx = (long)g.n[1] * g.n[2] * 8;
+  g.dx = (lx1[0] - lx0[0]) / (g.n[0] - 2 * c.Ng);
+  g.dy = (lx1[1] - lx0[1]) / (g.n[1] - 2 * c.Ng);
+  g.dz = (lx1[2] - lx0[2]) / (g.n[2] - 2 * c.Ng);
+  g.pp.gamma = c.gamma;
+  g.pp.pfloor = c.pfloor;
+  g.pp.theta = c.theta;
+  g.pp.IS = c.IS;
+  g.pp.sz10A = c.sz10A;
+}
+
+struct Acc {
+  double max_lambda = 0.0;
+  long hlld_success = 0, hlld_failure = 0, c2p_iter = 0, c2p_solves = 0;
+  long retries = 0, failures = 0;
+};
+
+/* GodunovOperator::ConsToPrim, Mara/hydro.cpp:125-202. U is synchronized first (the
+ * reference does so in place, through a const_cast). Returns the failed-zone count. */
+int cons_to_prim_all(Grid &g, double *U, double *P, Acc &acc)
+{
+  synchronize(g, U);
+  const long nz = g.T;
+  long ttl = 0, iters = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : ttl, iters)
+  for (long z = 0; z < nz; ++z) {
+    std::int64_t it = 0;
+    int err = (cons_to_prim(U + 8 * z, P + 8 * z, g.pp.gamma, g.pp.pfloor, &it) != 0);
+    g.failure_mask[z] = err;
+    ttl += err;
+    iters += it;
+  }
+  acc.c2p_iter += iters;
+  acc.c2p_solves += nz;
+  if (ttl && g.cfg.ereset) {
+#pragma omp parallel for schedule(static)
+    for (long z = 0; z < nz; ++z) {
+      if (g.failure_mask[z]) {
+        double *p = P + 8 * z;
+        p[0] = g.cfg.dreset;
+        p[1] = g.cfg.preset;
+        p[2] = p[3] = p[4] = 0.0;
+        p[5] = U[8 * z + 5];
+        p[6] = U[8 * z + 6];
+        p[7] = U[8 * z + 7];
+        prim_to_cons(p, U + 8 * z, g.pp.gamma);
+      }
+    }
+    std::fill(g.failure_mask.begin(), g.failure_mask.end(), 0);
+    ttl = 0;
+  }
+  return (int)ttl;
+}
+
+/* MethodOfLinesSplit::intercell_flux_sweep, Mara/plm-split.cpp:43-74 */
+void flux_sweep(Grid &g, const double *P, double *F, int dim, Acc &acc)
+{
+  const long S = dim == 1 ? g.sx : dim == 2 ? g.sy : g.sz;
+  const long end = 8 * g.T - 3 * S;
+  const int recon = g.cfg.recon;
+  const int solver = g.cfg.riemann;
+  double lam = 0.0;
+  long succ = 0, fail = 0, iters = 0, retries = 0, failures = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : lam) \
+    reduction(+ : succ, fail, iters, retries, failures)
+  for (long i = 2 * S; i < end; i += 8) {
+    double Pl[8], Pr[8];
+    for (int q = 0; q < 8; ++q) {
+      const long m = i + q;
+      double v[6] = { P[m-2*S], P[m-S], P[m+0], P[m+1*S], P[m+2*S], P[m+3*S] };
+      switch (recon) {
+      case RECON_PCM: Pl[q] = v[2]; Pr[q] = v[3]; break;
+      case RECON_PLM:
+        Pl[q] = reconstruct(&v[2], PLM_C2R, g.pp);
+        Pr[q] = reconstruct(&v[3], PLM_C2L, g.pp);
+        break;
+      default:
+        Pl[q] = reconstruct(&v[2], WENO5_FV_C2R, g.pp);
+        Pr[q] = reconstruct(&v[3], WENO5_FV_C2L, g.pp);
+        break;
+      }
+    }
+    Tally<double> t;
+    int error = riemann_flux(solver, Pl, Pr, &F[i], dim, g.pp, t);
+    if (error) {
+      ++retries;
+      error = riemann_flux(solver, &P[i], &P[i + S], &F[i], dim, g.pp, t);
+      if (error) {
+        ++failures;
+        for (int q = 0; q < 8; ++q) F[i + q] = 0.0;  // the reference leaves this memory unwritten
+      }
+    }
+    if (t.max_lambda > lam) lam = t.max_lambda;
+    succ += t.hlld_success;
+    fail += t.hlld_failure;
+    iters += t.c2p_iterations;
+  }
+  if (lam > acc.max_lambda) acc.max_lambda = lam;
+  acc.hlld_success += succ;
+  acc.hlld_failure += fail;
+  acc.c2p_iter += iters;
+  acc.retries += retries;
+  acc.failures += failures;
+}
+
+/* Rmhd::ConstrainedTransport3d, Mara/rmhd.cpp:293-337. The reference's temporaries are
+ * uninitialised outside [sx, stride0-sx); those zones are ghosts, zero is used here. */
+void constrained_transport_3d(const Grid &g, double *Fx, double *Fy, double *Fz)
+{
+  const long nzone = g.T;
+  std::vector<double> FxBy(nzone, 0.0), FxBz(nzone, 0.0), FyBz(nzone, 0.0), FyBx(nzone, 0.0),
+      FzBx(nzone, 0.0), FzBy(nzone, 0.0);
+  const long sx = g.sx, sy = g.sy, sz = g.sz, s0 = 8 * g.T;
+#pragma omp parallel for schedule(static)
+  for (long i = sx; i < s0 - sx; i += 8) {
+    const double *F, *G, *H;
+    F = &Fx[By + i];
+    G = &Fy[Bx + i];
+    FxBy[i/8] = (2*F[0]+F[sy]+F[-sy]-G[0]-G[sx]-G[-sy]-G[ sx-sy])*0.125;
+    FyBx[i/8] = (2*G[0]+G[sx]+G[-sx]-F[0]-F[sy]-F[-sx]-F[-sx+sy])*0.125;
+    G = &Fy[Bz + i];
+    H = &Fz[By + i];
+    FyBz[i/8] = (2*G[0]+G[sz]+G[-sz]-H[0]-H[sy]-H[-sz]-H[ sy-sz])*0.125;
+    FzBy[i/8] = (2*H[0]+H[sy]+H[-sy]-G[0]-G[sz]-G[-sy]-G[-sy+sz])*0.125;
+    H = &Fz[Bx + i];
+    F = &Fx[Bz + i];
+    FzBx[i/8] = (2*H[0]+H[sx]+H[-sx]-F[0]-F[sz]-F[-sx]-F[ sz-sx])*0.125;
+    FxBz[i/8] = (2*F[0]+F[sz]+F[-sz]-H[0]-H[sx]-H[-sz]-H[-sz+sx])*0.125;
+  }
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < s0; i += 8) {
+    Fx[i+Bx] = 0.0;        Fx[i+By] = FxBy[i/8];  Fx[i+Bz] = FxBz[i/8];
+    Fy[i+Bx] = FyBx[i/8];  Fy[i+By] = 0.0;        Fy[i+Bz] = FyBz[i/8];
+    Fz[i+Bx] = FzBx[i/8];  Fz[i+By] = FzBy[i/8];  Fz[i+Bz] = 0.0;
+  }
+}
+
+/* MethodOfLinesSplit::dUdt, Mara/plm-split.cpp:11-33 and drive_sweeps_3d :103-120.
+ * Returns false on IntermediateFailure (c2p failed somewhere). */
+bool dUdt(Grid &g, const std::vector<double> &Uin, std::vector<double> &L, Acc &acc, int *err)
+{
+  std::vector<double> U = Uin;
+  L.assign(U.size(), 0.0);
+  std::vector<double> &P = g.prim_array;
+  int e = cons_to_prim_all(g, U.data(), P.data(), acc);
+  if (e != 0) { *err = e; return false; }
+  const long s0 = 8 * g.T;
+  std::vector<double> F(s0, 0.0), G(s0, 0.0), H(s0, 0.0);
+  flux_sweep(g, P.data(), F.data(), 1, acc);
+  flux_sweep(g, P.data(), G.data(), 2, acc);
+  flux_sweep(g, P.data(), H.data(), 3, acc);
+  constrained_transport_3d(g, F.data(), G.data(), H.data());
+  const long sx = g.sx, sy = g.sy, sz = g.sz;
+  const double dx = g.dx, dy = g.dy, dz = g.dz;
+#pragma omp parallel for schedule(static)
+  for (long i = sx; i < s0; ++i) {
+    L[i] = -(F[i]-F[i-sx])/dx - (G[i]-G[i-sy])/dy - (H[i]-H[i-sz])/dz;
+  }
+  return true;
+}
+
+/* RungeKutta*::AdvanceState, Mara/runge-kutta.hpp:7-55 */
+bool advance_state(Grid &g, std::vector<double> &U, double dt, Acc &acc, int *err)
+{
+  const size_t n = U.size();
+  std::vector<double> L, U1(n);
+  switch (g.cfg.rk) {
+  case 1:
+    if (!dUdt(g, U, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) U[i] += dt*L[i];
+    return true;
+  case 2:
+    if (!dUdt(g, U, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) U1[i] = U[i] + dt*L[i];
+    if (!dUdt(g, U1, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) U[i] = 0.5*(U[i] + U1[i] + dt*L[i]);
+    return true;
+  case 4: {
+    std::vector<double> L1(n), L2(n), L3(n), T(n);
+    if (!dUdt(g, U, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) L1[i] = dt*L[i];
+    for (size_t i = 0; i < n; ++i) T[i] = U[i] + 0.5*L1[i];
+    if (!dUdt(g, T, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) L2[i] = dt*L[i];
+    for (size_t i = 0; i < n; ++i) T[i] = U[i] + 0.5*L2[i];
+    if (!dUdt(g, T, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) L3[i] = dt*L[i];
+    for (size_t i = 0; i < n; ++i) T[i] = U[i] + 1.0*L3[i];
+    if (!dUdt(g, T, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) U[i] += (1.0/6.0)*(L1[i] + 2.0*L2[i] + 2.0*L3[i] + dt*L[i]);
+    return true;
+  }
+  default:
+    if (!dUdt(g, U, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) U1[i] = U[i] + dt*L[i];
+    if (!dUdt(g, U1, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) U1[i] = 3./4*U[i] + 1./4*U1[i] + 1./4*dt*L[i];
+    if (!dUdt(g, U1, L, acc, err)) return false;
+    for (size_t i = 0; i < n; ++i) U[i] = 1./3*U[i] + 2./3*U1[i] + 2./3*dt*L[i];
+    return true;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void oracle_set_threads(int n)
+{
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+int oracle_get_threads(void)
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void *oracle_create(const oracle_config *cfg)
+{
+  Grid *g = new Grid;
+  g->cfg = *cfg;
+  prepare(*g);
+  g->max_lambda = 0.0;
+  g->prim_array.assign(8 * g->T, 0.0);
+  g->failure_mask.assign(g->T, 0);
+  std::memset(&g->last, 0, sizeof(g->last));
+  return g;
+}
+
+void oracle_destroy(void *h) { delete static_cast<Grid *>(h); }
+
+long oracle_num_zones(void *h) { return static_cast<Grid *>(h)->T; }
+
+/* luaC_Mara_advance, Mara/mara.cpp:297-341: returns the error count; P_user is only
+ * overwritten when it is zero. */
+int oracle_advance(void *h, double *P_user, double dt, oracle_stats *stats)
+{
+  Grid &g = *static_cast<Grid *>(h);
+  const long s0 = 8 * g.T;
+  std::vector<double> P(P_user, P_user + s0), U(s0, 0.0);
+  Acc acc;
+  g.max_lambda = 0.0;
+  std::fill(g.prim_array.begin(), g.prim_array.end(), 0.0);
+  synchronize(g, P.data());
+#pragma omp parallel for schedule(static)
+  for (long z = 0; z < g.T; ++z) prim_to_cons(&P[8 * z], &U[8 * z], g.pp.gamma);
+
+  int errors = 0;
+  int ierr = 0;
+  if (advance_state(g, U, dt, acc, &ierr)) {
+    P = g.prim_array;
+    errors = cons_to_prim_all(g, U.data(), P.data(), acc);
+  }
+  else {
+    errors = ierr;
+  }
+  if (errors == 0) std::memcpy(P_user, P.data(), s0 * sizeof(double));
+  g.max_lambda = acc.max_lambda;
+  g.last.max_lambda = acc.max_lambda;
+  g.last.hlld_success = (double)acc.hlld_success;
+  g.last.hlld_failure = (double)acc.hlld_failure;
+  g.last.c2p_iterations = (double)acc.c2p_iter;
+  g.last.c2p_solves = (double)acc.c2p_solves;
+  g.last.riemann_retries = (double)acc.retries;
+  g.last.riemann_failures = (double)acc.failures;
+  if (stats) *stats = g.last;
+  return errors;
+}
+
+/* luaC_Mara_get_timestep, Mara/mara.cpp:374-381 (inf when no solve has run) */
+double oracle_get_timestep(void *h, double cfl)
+{
+  Grid &g = *static_cast<Grid *>(h);
+  return cfl * g.min_dx / g.max_lambda;
+}
+
+/* luaC_mean_max_divB, Mara/mara.cpp:2264-2324, on a primitive array (set_primitive).
+ * Returns the reference's (mean, signed max) pair and, in absmax, max |div B| / dx. */
+void oracle_mean_max_divB(void *h, const double *P, double *mean, double *max, double *absmax)
+{
+  Grid &g = *static_cast<Grid *>(h);
+  const int Ng = g.cfg.Ng;
+  const int *n = g.n;
+  const long T = g.T;
+  std::vector<double> div(T, 0.0);
+  auto M = [&](long i, long j, long k) { return ((i * n[1] + j) * n[2] + k) * 8; };
+  const int bx = 5, by = 6, bz = 7;
+#pragma omp parallel for schedule(static)
+  for (long i = Ng; i < n[0]-Ng-1; ++i) {
+    for (long j = Ng; j < n[1]-Ng-1; ++j) {
+      for (long k = Ng; k < n[2]-Ng-1; ++k) {
+        const double *fx = P + bx, *fy = P + by, *fz = P + bz;
+        div[M(i,j,k)/8] =
+            ((fx[M(i+1,j  ,k  )] + fx[M(i+1,j+1,k  )] + fx[M(i+1,j  ,k+1)] + fx[M(i+1,j+1,k+1)]) -
+             (fx[M(i  ,j  ,k  )] + fx[M(i  ,j+1,k  )] + fx[M(i  ,j  ,k+1)] + fx[M(i  ,j+1,k+1)])) / 4.0
+          + ((fy[M(i  ,j+1,k  )] + fy[M(i  ,j+1,k+1)] + fy[M(i+1,j+1,k  )] + fy[M(i+1,j+1,k+1)]) -
+             (fy[M(i  ,j  ,k  )] + fy[M(i  ,j  ,k+1)] + fy[M(i+1,j  ,k  )] + fy[M(i+1,j  ,k+1)])) / 4.0
+          + ((fz[M(i  ,j  ,k+1)] + fz[M(i+1,j  ,k+1)] + fz[M(i  ,j+1,k+1)] + fz[M(i+1,j+1,k+1)]) -
+             (fz[M(i  ,j  ,k  )] + fz[M(i+1,j  ,k  )] + fz[M(i  ,j+1,k  )] + fz[M(i+1,j+1,k  )])) / 4.0;
+      }
+    }
+  }
+  double sum = 0.0, mx = div[0], amx = 0.0;
+  for (long z = 0; z < T; ++z) {   // std::valarray::sum / max: sequential order
+    if (z > 0) sum += div[z]; else sum = div[0];
+    if (div[z] > mx) mx = div[z];
+    if (std::fabs(div[z]) > amx) amx = std::fabs(div[z]);
+  }
+  const double glb = (double)g.cfg.N[0] * g.cfg.N[1] * g.cfg.N[2];
+  if (mean) *mean = sum / T / glb;
+  if (max) *max = mx;
+  if (absmax) *absmax = amx / g.min_dx;
+}
+
+/* ---------------------------------------------------------------------------
+ * Point functions, batched
+ * ------------------------------------------------------------------------- */
+void oracle_prim_to_cons_batch(long n, const double *P, double *U, double gamma, int *err)
+{
+#pragma omp parallel for schedule(static)
+  for (long z = 0; z < n; ++z) {
+    int e = prim_to_cons(P + 8 * z, U + 8 * z, gamma);
+    if (err) err[z] = e;
+  }
+}
+
+/* Rmhd::ConsToPrim gamma-law cascade; P holds the previous prim state (guess) on entry */
+void oracle_cons_to_prim_batch(long n, const double *U, double *P, double gamma, double pfloor,
+                               int *err, int *iters)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long z = 0; z < n; ++z) {
+    std::int64_t it = 0;
+    int e = cons_to_prim(U + 8 * z, P + 8 * z, gamma, pfloor, &it);
+    if (err) err[z] = e;
+    if (iters) iters[z] = (int)it;
+  }
+}
+
+/* Single anton2dzw / noble1dw solves from the cons-based estimate (mara.cpp:1819-1879) */
+void oracle_c2p_solver_batch(long n, int which, const double *U, double *P, double gamma,
+                             double pfloor, int *err, int *iters)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long z = 0; z < n; ++z) {
+    C2P<double> s(gamma, pfloor);
+    s.new_state(U + 8 * z);
+    s.estimate_from_cons();
+    int e = which == 0 ? s.solve_anton2dzw(P + 8 * z) : s.solve_noble1dw(P + 8 * z);
+    if (err) err[z] = e;
+    if (iters) iters[z] = s.iterations;
+  }
+}
+
+void oracle_flux_eigen_batch(long n, const double *P, double *F, double *ap, double *am,
+                             int dim, double gamma)
+{
+#pragma omp parallel for schedule(static)
+  for (long z = 0; z < n; ++z) {
+    double U[8];
+    prim_to_cons(P + 8 * z, U, gamma);
+    flux_and_eigenvalues(U, P + 8 * z, F + 8 * z, ap + z, am + z, dim, gamma);
+  }
+}
+
+int oracle_quartic(double d4, double d3, double d2, double d1, double d0, double *roots)
+{
+  return solve_quartic(d4, d3, d2, d1, d0, roots);
+}
+
+void oracle_reconstruct_batch(long n, const double *V, double *out, int op, double theta,
+                              int IS, double sz10A)
+{
+  PhysParams pp;
+  pp.theta = theta;
+  pp.IS = IS;
+  pp.sz10A = sz10A;
+  for (long z = 0; z < n; ++z) out[z] = reconstruct(V + 6 * z + 2, op, pp);
+}
+
+/* One Riemann solve per pair; lam receives each solve's max |lambda| (0 if none) */
+void oracle_riemann_batch(long n, int solver, const double *PL, const double *PR, double *F,
+                          int dim, double gamma, double pfloor, int *err, double *lam,
+                          int *hlld_ok)
+{
+  PhysParams pp;
+  pp.gamma = gamma;
+  pp.pfloor = pfloor;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long z = 0; z < n; ++z) {
+    Tally<double> t;
+    for (int q = 0; q < 8; ++q) F[8 * z + q] = 0.0;
+    int e = riemann_flux(solver, PL + 8 * z, PR + 8 * z, F + 8 * z, dim, pp, t);
+    if (err) err[z] = e;
+    if (lam) lam[z] = t.max_lambda;
+    if (hlld_ok) hlld_ok[z] = (int)t.hlld_success;
+  }
+}
+
+/* Flop counts on the oracle's own path with the Counted scalar (same roundings):
+ * reconstruction + Riemann solve per interface over `nsample` interfaces of each axis
+ * spread evenly through the reference's sweep range, and the c2p cascade (zero
+ * previous-prim guess, as in the first RK stage) over `nsample` zones.
+ * out[0..2] = mean flops per interface for x, y, z; out[3] = mean c2p flops per zone;
+ * out[4..6] = mean transcendental calls per interface; out[7] = c2p transcendentals. */
+void oracle_count_flops(void *h, const double *P_synced, long nsample, double *out)
+{
+  Grid &g = *static_cast<Grid *>(h);
+  const long s0 = 8 * g.T;
+  for (int dim = 1; dim <= 3; ++dim) {
+    const long S = dim == 1 ? g.sx : dim == 2 ? g.sy : g.sz;
+    const long first = 2 * S / 8, last = (s0 - 3 * S) / 8;  // zone range
+    const long span = last - first;
+    FlopTally total;
+    for (long s = 0; s < nsample; ++s) {
+      const long zone = first + (span * s) / nsample;
+      const long i = 8 * zone;
+      g_tally = FlopTally();
+      Counted Pl[8], Pr[8], F[8];
+      for (int q = 0; q < 8; ++q) {
+        const long m = i + q;
+        Counted v[6] = { P_synced[m-2*S], P_synced[m-S], P_synced[m], P_synced[m+S],
+                         P_synced[m+2*S], P_synced[m+3*S] };
+        switch (g.cfg.recon) {
+        case RECON_PCM: Pl[q] = v[2]; Pr[q] = v[3]; break;
+        case RECON_PLM:
+          Pl[q] = reconstruct(&v[2], PLM_C2R, g.pp); Pr[q] = reconstruct(&v[3], PLM_C2L, g.pp); break;
+        default:
+          Pl[q] = reconstruct(&v[2], WENO5_FV_C2R, g.pp); Pr[q] = reconstruct(&v[3], WENO5_FV_C2L, g.pp); break;
+        }
+      }
+      Tally<Counted> t;
+      if (riemann_flux(g.cfg.riemann, Pl, Pr, F, dim, g.pp, t)) {
+        Counted a[8], b[8];
+        for (int q = 0; q < 8; ++q) { a[q] = P_synced[i + q]; b[q] = P_synced[i + S + q]; }
+        riemann_flux(g.cfg.riemann, a, b, F, dim, g.pp, t);
+      }
+      total.add += g_tally.add; total.mul += g_tally.mul; total.div += g_tally.div;
+      total.sqrt += g_tally.sqrt; total.trans += g_tally.trans;
+    }
+    out[dim - 1] = (double)total.flops() / nsample;
+    out[3 + dim] = (double)total.trans / nsample;
+  }
+  FlopTally total;
+  const long nzone = g.T;
+  for (long s = 0; s < nsample; ++s) {
+    const long z = (nzone * s) / nsample;
+    Counted P[8], U[8], P0[8];
+    for (int q = 0; q < 8; ++q) P[q] = P_synced[8 * z + q];
+    prim_to_cons(P, U, g.pp.gamma);
+    g_tally = FlopTally();
+    std::int64_t it = 0;
+    cons_to_prim(U, P0, g.pp.gamma, g.pp.pfloor, &it);
+    total.add += g_tally.add; total.mul += g_tally.mul; total.div += g_tally.div;
+    total.sqrt += g_tally.sqrt; total.trans += g_tally.trans;
+  }
+  out[3] = (double)total.flops() / nsample;
+  out[7] = (double)total.trans / nsample;
+}
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+/* ORACLE — test infrastructure only.
+ *
+ * C-ABI harness over the compiled reference (Mara's own sources, built from where they
+ * lie under the reference tree by oracle/Makefile into oracle/_ref/). It stands in for the
+ * Lua host: the set-up sequence of ctf-core/run-mara.lua:113-148 and the body of
+ * luaC_Mara_advance (Mara/mara.cpp:297-341) are restated here against the reference's
+ * classes, so every number it returns is computed by the reference's code.
+ * Function names and arguments match liboracle.so (prefix ref_ instead of oracle_).
+ */
+#include "mara.hpp"
+
+#include <cstring>
+#include <valarray>
+
+extern "C" {
+
+typedef struct {
+  int N[3];
+  int Ng;
+  double x0[3], x1[3];
+  double gamma;
+  int recon;
+  double theta;
+  int IS;
+  double sz10A;
+  int riemann;
+  int rk;
+  double pfloor;
+  int ereset;
+  double preset, dreset;
+} oracle_config;
+
+typedef struct {
+  double max_lambda;
+  double hlld_success;
+  double hlld_failure;
+  double c2p_iterations;
+  double c2p_solves;
+  double riemann_retries;
+  double riemann_failures;
+  double reserved;
+} oracle_stats;
+
+}  // extern "C"
+
+static MaraApplication *App = nullptr;
+
+static void ref_start(double gamma)
+{
+  if (App) delete App;
+  App = new MaraApplication;
+  HydroModule::Mara = App;
+  delete App->fluid;
+  App->fluid = new AdiabaticIdealRmhd;
+  delete App->eos;
+  App->eos = new AdiabaticEos(gamma);
+  rmhd_c2p_set_gamma(gamma);
+}
+
+static void ref_params(double theta, int IS, double sz10A, double pfloor)
+{
+  reconstruct_set_plm_theta(theta);
+  reconstruct_set_smoothness_indicator(IS == 1 ? ImprovedBorges08
+                                       : IS == 2 ? ImprovedShenZha10 : OriginalJiangShu96);
+  reconstruct_set_shenzha10_A(sz10A);
+  rmhd_c2p_set_pressure_floor(pfloor);
+}
+
+static RiemannSolver *make_riemann(int which)
+{
+  if (which == 0) return new HllRiemannSolver;
+  if (which == 1) return new HllcRiemannSolver;
+  return new HlldRmhdRiemannSolver;
+}
+
+extern "C" {
+
+void *ref_create(const oracle_config *c)
+{
+  ref_start(c->gamma);
+  App->godunov = new MethodOfLinesSplit;
+  App->boundary = new PeriodicBoundary;
+  App->riemann = make_riemann(c->riemann);
+  switch (c->rk) {
+  case 1: App->advance = new RungeKuttaSingleStep; break;
+  case 2: App->advance = new RungeKuttaRk2Tvd; break;
+  case 4: App->advance = new RungeKuttaClassicRk4; break;
+  default: App->advance = new RungeKuttaShuOsherRk3; break;
+  }
+  GodunovOperator::reconstruct_method =
+      c->recon == 0 ? GodunovOperator::RECONSTRUCT_PCM
+      : c->recon == 1 ? GodunovOperator::RECONSTRUCT_PLM : GodunovOperator::RECONSTRUCT_WENO5;
+  ref_params(c->theta, c->IS, c->sz10A, c->pfloor);
+  if (c->ereset) App->godunov->enable_emergency_reset();
+  else App->godunov->disable_emergency_reset();
+  App->godunov->set_reset_pressure(c->preset);
+  App->godunov->set_reset_density(c->dreset);
+  App->domain = new SimpleCartesianDomain(c->x0, c->x1, c->N, 3, 8, c->Ng);
+  RiemannSolver::ResetMaxLambda();
+  return App;
+}
+
+void ref_destroy(void *) {}
+
+long ref_num_zones(void *) { return App->domain->GetNumberOfZones(); }
+
+int ref_advance(void *, double *UserPrim, double dt, oracle_stats *stats)
+{
+  const unsigned N = App->domain->GetNumberOfZones() * App->domain->get_Nq();
+  std::valarray<double> P(N);
+  std::valarray<double> U(N);
+  int errors;
+  const int s0 = HlldRmhdRiemannSolver::NumberOfSuccess;
+  const int f0 = HlldRmhdRiemannSolver::NumberOfFailure;
+
+  std::memcpy(&P[0], UserPrim, N * sizeof(double));
+  RiemannSolver::ResetMaxLambda();
+  App->FailureMask.resize(App->domain->GetNumberOfZones());
+  App->PrimitiveArray.resize(P.size());
+  App->godunov->PrimToCons(P, U);
+  try {
+    App->advance->AdvanceState(U, dt);
+    errors = App->godunov->ConsToPrim(U, P);
+  }
+  catch (const GodunovOperator::IntermediateFailure &e) {
+    errors = App->FailureMask.sum();
+  }
+  if (errors == 0) std::memcpy(UserPrim, &P[0], N * sizeof(double));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->max_lambda = RiemannSolver::GetMaxLambda();
+    stats->hlld_success = HlldRmhdRiemannSolver::NumberOfSuccess - s0;
+    stats->hlld_failure = HlldRmhdRiemannSolver::NumberOfFailure - f0;
+  }
+  return errors;
+}
+
+double ref_get_timestep(void *, double cfl)
+{
+  return cfl * App->domain->get_min_dx() / RiemannSolver::GetMaxLambda();
+}
+
+void ref_prim_to_cons_batch(long n, const double *P, double *U, double gamma, int *err)
+{
+  ref_start(gamma);
+  for (long z = 0; z < n; ++z) {
+    int e = App->fluid->PrimToCons(P + 8 * z, U + 8 * z);
+    if (err) err[z] = e;
+  }
+}
+
+void ref_cons_to_prim_batch(long n, const double *U, double *P, double gamma, double pfloor,
+                            int *err, int *iters)
+{
+  ref_start(gamma);
+  rmhd_c2p_set_pressure_floor(pfloor);
+  for (long z = 0; z < n; ++z) {
+    int e = App->fluid->ConsToPrim(U + 8 * z, P + 8 * z);
+    if (err) err[z] = e;
+    if (iters) iters[z] = rmhd_c2p_get_iterations();
+  }
+}
+
+void ref_c2p_solver_batch(long n, int which, const double *U, double *P, double gamma,
+                          double pfloor, int *err, int *iters)
+{
+  rmhd_c2p_set_gamma(gamma);
+  rmhd_c2p_set_pressure_floor(pfloor);
+  for (long z = 0; z < n; ++z) {
+    rmhd_c2p_new_state(U + 8 * z);
+    rmhd_c2p_estimate_from_cons();
+    int e = which == 0 ? rmhd_c2p_solve_anton2dzw(P + 8 * z) : rmhd_c2p_solve_noble1dw(P + 8 * z);
+    if (err) err[z] = e;
+    if (iters) iters[z] = rmhd_c2p_get_iterations();
+  }
+}
+
+void ref_flux_eigen_batch(long n, const double *P, double *F, double *ap, double *am, int dim,
+                          double gamma)
+{
+  ref_start(gamma);
+  for (long z = 0; z < n; ++z) {
+    double U[8];
+    App->fluid->PrimToCons(P + 8 * z, U);
+    App->fluid->FluxAndEigenvalues(U, P + 8 * z, F + 8 * z, ap + z, am + z, dim);
+  }
+}
+
+int ref_quartic(double d4, double d3, double d2, double d1, double d0, double *roots)
+{
+  QuarticEquation q(d4, d3, d2, d1, d0);
+  return q.Solve(roots);
+}
+
+void ref_reconstruct_batch(long n, const double *V, double *out, int op, double theta, int IS,
+                           double sz10A)
+{
+  ref_params(theta, IS, sz10A, -1.0);
+  for (long z = 0; z < n; ++z) out[z] = reconstruct(V + 6 * z + 2, (ReconstructOperation)op);
+}
+
+void ref_riemann_batch(long n, int solver, const double *PL, const double *PR, double *F,
+                       int dim, double gamma, double pfloor, int *err, double *lam, int *hlld_ok)
+{
+  ref_start(gamma);
+  rmhd_c2p_set_pressure_floor(pfloor);
+  RiemannSolver *rs = make_riemann(solver);
+  for (long z = 0; z < n; ++z) {
+    RiemannSolver::ResetMaxLambda();
+    const int s0 = HlldRmhdRiemannSolver::NumberOfSuccess;
+    for (int q = 0; q < 8; ++q) F[8 * z + q] = 0.0;
+    int e = rs->IntercellFlux(PL + 8 * z, PR + 8 * z, 0, F + 8 * z, 0.0, dim);
+    if (err) err[z] = e;
+    if (lam) lam[z] = RiemannSolver::GetMaxLambda();
+    if (hlld_ok) hlld_ok[z] = HlldRmhdRiemannSolver::NumberOfSuccess - s0;
+  }
+  delete rs;
+}
+
+}  // extern "C"
