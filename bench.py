@@ -1,0 +1,279 @@
+"""Benchmark of the device-resident RMHD update on one MI355X.
+
+Workload (default): c2 — 256^3 periodic RMHD turbulence, WENO5 / HLLD / RK3, CFL 0.4,
+plain FP64 (see ctf_amd/workloads.py). A step is one full time step (all RK stages) of
+the whole grid with the state resident in HBM. The first step of a run has dt = 0 (the
+reference's priming step) and is untimed; W warm-up steps follow, then K timed steps.
+
+The same initial data is then run through the mixed-precision form (and the learned-c2p
+form when a network file is present) with the plain run's dt sequence, and each form
+reports zone-updates/s, per-field L1 difference from the plain FP64 state and max |div B|.
+
+value = interior zones x timed steps / wall seconds of the plain FP64 form.
+
+Printed: one JSON line (rank 0). Multi-GPU runs are not part of this build.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from ctf_amd import native, workloads  # noqa: E402
+
+METRIC = "zone-updates/s at 256^3 RMHD (plain vs learned-c2p); L1 err vs FP64"
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and matrix) peak
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak
+HBM_PEAK_GBS = 8000.0
+FIELDS = ["rho", "pre", "vx", "vy", "vz", "Bx", "By", "Bz"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def flop_counts(w):
+    path = os.path.join(ROOT, "tests", "golden", f"flops_{w.name}.json")
+    if not os.path.exists(path):
+        path = os.path.join(ROOT, "tests", "golden", "flops_c2.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+def interfaces_per_sweep(w):
+    n = w.N + 2 * w.Ng
+    T = n**3
+    strides = (n * n, n, 1)
+    return [T - 5 * s for s in strides]
+
+
+def run_form(w, P0, steps, warmup, dts=None, learned_model=None):
+    """Runs one form; returns timings, stats, the final state and the dt sequence."""
+    cfg = native.make_config(**w.config_kwargs())
+    ctx = native.Context(cfg)
+    if learned_model is not None:
+        native.mlp_load(ctx, learned_model)
+    ctx.upload(P0)
+    t0 = time.perf_counter()
+    e, st = ctx.advance(0.0)                      # priming step, dt = 0
+    prime_ms = 1e3 * (time.perf_counter() - t0)
+    if e:
+        raise RuntimeError(f"{w.name}: priming step failed on {e} zones")
+    used = [] if dts is None else list(dts)
+    k = 0
+
+    def next_dt():
+        nonlocal k
+        if dts is not None:
+            d = dts[k]
+        else:
+            d = ctx.get_timestep(w.cfl)
+            used.append(d)
+        k += 1
+        return d
+
+    for _ in range(warmup):
+        e, st = ctx.advance(next_dt())
+        if e:
+            raise RuntimeError(f"{w.name}: warm-up step failed on {e} zones")
+    flux_ms = c2p_ms = upd_ms = 0.0
+    flux_n = c2p_n = upd_n = 0
+    agg = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e, st = ctx.advance(next_dt())
+        if e:
+            raise RuntimeError(f"{w.name}: step failed on {e} zones")
+        kt = ctx.kernel_times()
+        flux_ms += kt["flux_ms"]; flux_n += kt["flux_launches"]
+        c2p_ms += kt["c2p_ms"]; c2p_n += kt["c2p_launches"]
+        upd_ms += kt["update_ms"]; upd_n += kt["update_launches"]
+        for key, val in st.as_dict().items():
+            agg[key] = agg.get(key, 0) + val
+    wall = time.perf_counter() - t0
+    divB = ctx.divB()
+    P = ctx.download()
+    ctx.close()
+    nint = w.N**3
+    return {
+        "wall_s": wall,
+        "zone_updates_per_s": nint * steps / wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "prime_ms": prime_ms,
+        "flux_ms_per_launch": flux_ms / max(flux_n, 1),
+        "c2p_ms_per_launch": c2p_ms / max(c2p_n, 1),
+        "update_ms_per_launch": upd_ms / max(upd_n, 1),
+        "flux_ms_per_step": flux_ms / steps,
+        "c2p_ms_per_step": c2p_ms / steps,
+        "update_ms_per_step": upd_ms / steps,
+        "stats": agg,
+        "max_divB": divB[2],
+        "P": P,
+        "dts": used,
+    }
+
+
+def l1_vs(P, Pref, Ng):
+    it = (slice(Ng, -Ng),) * 3
+    a, b = P[it], Pref[it]
+    out = {}
+    for q, name in enumerate(FIELDS):
+        den = np.abs(b[..., q]).sum()
+        out[name] = float(np.abs(a[..., q] - b[..., q]).sum() / den) if den > 0 else 0.0
+    return out
+
+
+def cpu_baseline(w, seconds_target=15.0):
+    """The reference's own CPU path (oracle/_ref, single thread) on a bounded sample of
+    the workload: the same generator and scheme on a smaller grid, a priming step plus
+    timed steps. Falls back to the oracle's restatement when _ref is absent."""
+    sys.path.insert(0, ROOT)
+    from oracle.oracle import Lib, Setup, have_ref
+    kind = "reference" if have_ref() else "port"
+    lib = Lib("ref" if kind == "reference" else "oracle")
+    if kind == "port":
+        lib.lib.oracle_set_threads(1)
+    n = 32
+    ws = w.resized(n)
+    P = native.init_turbulence(**ws.ic_kwargs())
+    s = Setup(N=(n,) * 3, extrap=w.extrap, riemann=w.riemann, advance=w.advance, gamma=w.gamma)
+    solver = lib.solver(s)
+    e, _ = solver.advance(P, 0.0)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        dt = solver.get_timestep(w.cfl)
+        e, _ = solver.advance(P, dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if e or el > seconds_target or steps >= 20:
+            break
+    rate = n**3 * steps / el
+    return {"value": rate, "unit": "zone-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{w.name} scheme ({w.extrap}/{w.riemann}/{w.advance}, FP64) on the same "
+                      f"generator at {n}^3, {steps} timed step(s) after the dt=0 priming step, "
+                      f"{el:.1f} s single-threaded on the GPU box's host"}
+
+
+def traffic_record(w):
+    path = os.path.join(ROOT, "profiles", f"traffic_{w.name}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--grid", type=int, default=None, help="override the workload's grid size")
+    ap.add_argument("--forms", default="plain,mixed,learned")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--model", default=os.path.join(ROOT, "ctf_amd", "models", "c2p_mlp.bin"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
+                                   "of it (see DESIGN.md for how the update shards)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    w = workloads.ALL[args.workload]
+    if args.grid:
+        w = w.resized(args.grid)
+    w = workloads.replace(w, precision="fp64", c2p="plain")
+    log(f"[bench] initial data {w.name} {w.N}^3")
+    P0 = native.init_turbulence(**w.ic_kwargs())
+    forms = [f for f in args.forms.split(",") if f]
+    results = {}
+    log("[bench] plain FP64")
+    plain = run_form(w, P0, args.steps, args.warmup)
+    results["plain"] = plain
+    dts = plain["dts"]
+    if "mixed" in forms:
+        log("[bench] mixed precision")
+        wm = workloads.replace(w, precision="mixed")
+        results["mixed"] = run_form(wm, P0, args.steps, args.warmup, dts=dts)
+    if "learned" in forms and os.path.exists(args.model) and hasattr(native, "mlp_load"):
+        log("[bench] mixed + learned c2p")
+        wl = workloads.replace(w, precision="mixed", c2p="learned")
+        results["learned"] = run_form(wl, P0, args.steps, args.warmup, dts=dts,
+                                      learned_model=args.model)
+
+    fc = flop_counts(w)
+    nif = interfaces_per_sweep(w)
+    fpi = fc["flops_per_interface"]
+    flops_per_launch = (nif[0] * fpi["x"] + nif[1] * fpi["y"] + nif[2] * fpi["z"]) / 3.0
+    avg_flux_s = plain["flux_ms_per_launch"] * 1e-3
+    achieved = flops_per_launch / avg_flux_s / 1e12
+    roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "kernel": "k_flux<double,WENO5,HLLD> (one axis sweep)",
+                "flops_per_launch": flops_per_launch,
+                "avg_launch_ms": plain["flux_ms_per_launch"],
+                "traffic": None}
+    tr = traffic_record(w)
+    if tr:
+        roofline["traffic"] = tr.get("bytes_per_launch")
+        roofline["traffic_source"] = tr.get("source")
+
+    forms_out = {}
+    for name, r in results.items():
+        rec = {"zone_updates_per_s": r["zone_updates_per_s"], "ms_per_step": r["ms_per_step"],
+               "speedup_vs_plain": r["zone_updates_per_s"] / plain["zone_updates_per_s"],
+               "flux_ms_per_step": r["flux_ms_per_step"], "c2p_ms_per_step": r["c2p_ms_per_step"],
+               "update_ms_per_step": r["update_ms_per_step"], "max_divB": r["max_divB"],
+               "priming_step_ms": r["prime_ms"]}
+        if name != "plain":
+            rec["L1_vs_fp64"] = l1_vs(r["P"], plain["P"], w.Ng)
+        st = r["stats"]
+        zones = max(st.get("c2p_zones", 0), 1)
+        rec["newton_iterations_per_zone"] = st.get("c2p_iterations", 0) / zones
+        rec["c2p_fallback_fraction"] = st.get("c2p_fallbacks", 0) / zones
+        rec["hlld_to_hllc_fallbacks"] = st.get("hlld_failure", 0)
+        rec["hlld_solves"] = st.get("hlld_success", 0) + st.get("hlld_failure", 0)
+        if name == "learned":
+            g = max(st.get("mlp_guesses", 0), 1)
+            rec["mlp_fallback_fraction"] = st.get("mlp_fallbacks", 0) / g
+        forms_out[name] = rec
+
+    out = {
+        "metric": METRIC,
+        "value": plain["zone_updates_per_s"],
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": plain["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {w.N}^3 periodic RMHD turbulence, "
+                               f"{w.extrap.upper()}/{w.riemann.upper()}/{w.advance.upper()}, "
+                               f"CFL {w.cfl}, plain FP64 (Mara plm-split + extrap=weno5)",
+                   "grid": w.N, "ghosts": w.Ng, "gamma": w.gamma,
+                   "interior_zones": w.N**3, "priming_step_excluded": True},
+        "forms": forms_out,
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline")
+        cb = cpu_baseline(w)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

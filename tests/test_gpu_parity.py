@@ -6,9 +6,10 @@ uses the ROCm device math library for pow/acos/cos/sin, so agreement is to round
 not bitwise:
   * closed forms (PrimToCons, fluxes, reconstruction): 1e-11 relative per state;
   * c2p: identical error codes, 1e-10 relative (the Newton stop is a 1e-12 step);
-  * fast speeds: 1e-10 for 99.9 % of states. The quartic's extreme roots are
-    ill-conditioned near double roots (sqrt of a small discriminant), where 1-ulp input
-    changes move them by ~1e-8;
+  * fast speeds: 1e-10 where the reference's quartic is well conditioned; elsewhere
+    within 4x of how far the oracle's own result moves under 1-ulp input perturbations
+    (Ferrari's resolvent takes square roots of near-zero discriminants, and ~20 % of
+    states with W <= 20 move by > 1e-10, some across the |lambda| <= 1 clamp);
   * Riemann fluxes: identical error codes, 1e-9 relative for >= 99.5 % of interfaces.
     The reference's HLLD is itself discontinuous under 1e-15 input perturbations for a
     few interfaces in 10^3 (attempt ladder / HLLC fallback flips); every remaining
@@ -64,15 +65,30 @@ def test_cons_to_prim_pressure_floor(gpu):
     assert (P[ok, 1] >= float(g["pfloor"])).all()
 
 
-def test_flux_and_eigenvalues(gpu):
+def oracle_envelope(oracle_lib, P, dim, n=16, seed=0):
+    """How far the oracle's own fast speeds move under 1-ulp input perturbations."""
+    rng = np.random.default_rng(seed)
+    _, ap, am = oracle_lib.flux_eigen(P, dim, GAMMA)
+    env = np.zeros(len(P))
+    for _ in range(n):
+        Q = P * (1 + 2e-16 * rng.normal(size=P.shape))
+        _, ap2, am2 = oracle_lib.flux_eigen(Q, dim, GAMMA)
+        env = np.maximum(env, np.maximum(np.abs(ap2 - ap), np.abs(am2 - am)))
+    return env
+
+
+def test_flux_and_eigenvalues(gpu, oracle_lib):
     g = load_golden("kat_flux.npz")
     for dim in (1, 2, 3):
         F, ap, am = gpu.flux_eigen(g["P"], dim, GAMMA)
         assert rel_rows(F, g[f"F{dim}"]).max() < 1e-11
+        env = oracle_envelope(oracle_lib, g["P"], dim)
         for mine, ref in ((ap, g[f"ap{dim}"]), (am, g[f"am{dim}"])):
             d = np.abs(mine - ref)
-            assert np.mean(d < 1e-10) >= 0.999
-            assert d.max() < 1e-5
+            # well-conditioned states agree to round-off; elsewhere the device result
+            # stays within what an ulp of input change does to the reference itself
+            assert d[env < 1e-12].max() < 1e-10
+            assert (d <= 1e-10 + 4 * env).all(), np.sort(d - 4 * env)[-3:]
 
 
 def test_reconstruct(gpu):
@@ -189,7 +205,11 @@ def test_turbulence_steps_vs_oracle(gpu, oracle_lib, extrap, riemann, advance, N
     for _ in range(4):
         eo, sto = so.advance(Po, dt)
         eg, stg = ctx.advance(dt)
-        assert eo == eg == 0
+        assert eo == eg
+        if eo:
+            # the reference fails this step too (WENO5 + HLLC does at step 4 here):
+            # nothing is committed on either side
+            break
         assert abs(stg.max_lambda - sto.max_lambda) <= 1e-12 * sto.max_lambda
         assert abs(stg.hlld_success - sto.hlld_success) <= 2
         dt = so.get_timestep(0.4)
@@ -212,6 +232,8 @@ def test_divB_measure(gpu, oracle_lib):
     P = P0.copy()
     x = (np.arange(N + 6) - 3 + 0.5) / N
     P[..., 5] = x[:, None, None]
+    P[..., 6] = 0.0
+    P[..., 7] = 1.0
     ctx.upload(P)
     m_g, x_g, a_g = ctx.divB()
     assert abs(x_g - 1.0 / N) < 1e-14 and abs(a_g - 1.0) < 1e-11
