@@ -262,40 +262,48 @@ __global__ __launch_bounds__(kBlock) void k_flux(const double *__restrict__ P, F
   double lam_out = 0.0;
   unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
   if (z < T - 3 * S) {
-    R pl[8], pr[8], p0[8], p1[8];
+    R ql[8], qr[8];
     const R theta = (R)pp.theta;
     for (int q = 0; q < 8; ++q) {
       const double *Pq = P + q * T + z;
       const R v0 = (R)Pq[0], v1 = (R)Pq[S];
-      p0[q] = v0;
-      p1[q] = v1;
       if (recon == RECON_WENO5) {
         const R vm2 = (R)Pq[-2 * S], vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S], v3 = (R)Pq[3 * S];
-        pl[q] = weno5<R, true>(vm2, vm1, v0, v1, v2, pp.IS, (R)pp.sz10A);
-        pr[q] = weno5<R, false>(vm1, v0, v1, v2, v3, pp.IS, (R)pp.sz10A);
+        ql[q] = weno5<R, true>(vm2, vm1, v0, v1, v2, pp.IS, (R)pp.sz10A);
+        qr[q] = weno5<R, false>(vm1, v0, v1, v2, v3, pp.IS, (R)pp.sz10A);
       }
       else if (recon == RECON_PLM) {
         const R vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S];
-        pl[q] = plm<R>(vm1, v0, v1, R(+1.0), theta);
-        pr[q] = plm<R>(v0, v1, v2, R(-1.0), theta);
+        ql[q] = plm<R>(vm1, v0, v1, R(+1.0), theta);
+        qr[q] = plm<R>(v0, v1, v2, R(-1.0), theta);
       }
       else {
-        pl[q] = v0;
-        pr[q] = v1;
+        ql[q] = v0;
+        qr[q] = v1;
       }
     }
     R lam = R(0);
     FluxCounters fc = {0, 0, 0, 0};
     R F[8];
-    int err = riemann_dispatch<R, DIM>(SOLVER, pl, pr, F, (R)pp.gamma, pp.pfloor, lam, fc);
-    if (err) {
-      cnt[2] = 1;
-      err = riemann_dispatch<R, DIM>(SOLVER, p0, p1, F, (R)pp.gamma, pp.pfloor, lam, fc);
-      if (err) {
-        cnt[3] = 1;
+    int err = 1;
+    // one call site: the reconstructed states, then (Mara/plm-split.cpp:68-71) the
+    // zone-centred states of the two neighbours
+#pragma unroll Shape<R>::unroll
+    for (int attempt = 0; attempt < 2 && err; ++attempt) {
+      if (attempt == 1) {
+        cnt[2] = 1;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) F[q] = R(0);
+        for (int q = 0; q < 8; ++q) {
+          ql[q] = (R)P[q * T + z];
+          qr[q] = (R)P[q * T + z + S];
+        }
       }
+      err = riemann_dispatch<R, DIM>(SOLVER, ql, qr, F, (R)pp.gamma, pp.pfloor, lam, fc);
+    }
+    if (err) {
+      cnt[3] = 1;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) F[q] = R(0);
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) Fout[q * T + z] = (FT)F[q];

@@ -64,6 +64,12 @@ template <> struct Tol<float> {
   static constexpr float hllc_small_a = 1e-6f;
 };
 
+/* Code shape per precision: the FP64 solver is register-bound, so its repeated pieces
+ * (the two fast-speed evaluations, the three function evaluations of a Newton step, the
+ * c2p attempts) run as loops over one copy of the code; the FP32 solver fits the
+ * registers and keeps them unrolled for instruction-level parallelism. */
+template <typename R> struct Shape { static constexpr int unroll = sizeof(R) == 8 ? 1 : 8; };
+
 struct PhysParams {
   double gamma;
   double pfloor;
@@ -76,7 +82,7 @@ __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dfabs(double x) { return fabs(x); }
 __device__ __forceinline__ float dfabs(float x) { return fabsf(x); }
-__device__ __forceinline__ double dcbrt_pow(double x) { return pow(x, 1. / 3.); }
+__device__ __forceinline__ double dcbrt_pow(double x) { return cbrt(x); }
 __device__ __forceinline__ float dcbrt_pow(float x) { return cbrtf(x); }
 __device__ __forceinline__ double dacos(double x) { return acos(x); }
 __device__ __forceinline__ float dacos(float x) { return acosf(x); }
@@ -84,6 +90,21 @@ __device__ __forceinline__ double dcos(double x) { return cos(x); }
 __device__ __forceinline__ float dcos(float x) { return cosf(x); }
 __device__ __forceinline__ double dsin(double x) { return sin(x); }
 __device__ __forceinline__ float dsin(float x) { return sinf(x); }
+
+/* Reciprocals. A correctly rounded FP64 division costs about ten instructions on
+ * gfx950 (div_scale x2, rcp, five fma, div_fmas, div_fixup); the hardware reciprocal
+ * refined by two Newton steps costs five and is within an ulp. Where the refinement is
+ * not finite (x = 0, inf, NaN) the raw reciprocal is returned, so 1/0 stays inf. */
+__device__ __forceinline__ double rcp(double x)
+{
+  const double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  double r = fma(y, e, y);
+  e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  return r == r ? r : y;
+}
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <typename R> __device__ __forceinline__ bool is_nan(R x) { return x != x; }
 template <typename R> __device__ __forceinline__ bool is_inf(R x) { return dfabs(x) == R(INFINITY); }
@@ -118,17 +139,18 @@ template <typename R> __device__ __forceinline__ int prim_to_cons(const R *P, R 
   const R V2 = P[vx]*P[vx] + P[vy]*P[vy] + P[vz]*P[vz];
   const R B2 = P[Bx]*P[Bx] + P[By]*P[By] + P[Bz]*P[Bz];
   const R Bv = P[Bx]*P[vx] + P[By]*P[vy] + P[Bz]*P[vz];
-  const R W2 = R(1) / (R(1) - V2);
+  const R W2 = rcp(R(1) - V2);
   const R W = dsqrt(W2);
   const R b0 = W*Bv;
-  const R b2 = (B2 + b0*b0) / W2;
-  const R bx = (P[Bx] + b0 * W*P[vx]) / W;
-  const R by = (P[By] + b0 * W*P[vy]) / W;
-  const R bz = (P[Bz] + b0 * W*P[vz]) / W;
-  const R e = (P[pre] / gm1) / P[rho];
-  const R e_ = e + R(0.5) * b2 / P[rho];
+  const R b2 = (B2 + b0*b0) * (R(1) - V2);
+  const R iW = rcp(W), irho = rcp(P[rho]);
+  const R bx = (P[Bx] + b0 * W*P[vx]) * iW;
+  const R by = (P[By] + b0 * W*P[vy]) * iW;
+  const R bz = (P[Bz] + b0 * W*P[vz]) * iW;
+  const R e = (P[pre] * rcp(gm1)) * irho;
+  const R e_ = e + R(0.5) * b2 * irho;
   const R p_ = P[pre] + R(0.5) * b2;
-  const R h_ = R(1) + e_ + p_ / P[rho];
+  const R h_ = R(1) + e_ + p_ * irho;
   U[ddd] = P[rho] * W;
   U[tau] = P[rho] * h_ * W2 - p_ - b0*b0 - U[ddd];
   U[Sx] = P[rho] * h_ * W2 * P[vx] - b0*bx;
@@ -146,7 +168,8 @@ template <typename R> __device__ __forceinline__ int prim_to_cons(const R *P, R 
 template <typename R> __device__ __forceinline__ void quartic_extremes(R d4, R d3, R d2, R d1, R d0,
                                                                       R &rmax, R &rmin, int &nr)
 {
-  const R a3 = d3/d4, a2 = d2/d4, a1 = d1/d4, a0 = d0/d4;
+  const R id4 = rcp(d4);
+  const R a3 = d3*id4, a2 = d2*id4, a1 = d1*id4, a0 = d0*id4;
   const R au2 = -a2;
   const R au1 = (a1*a3 - R(4.0)*a0);
   const R au0 = R(4.0)*a0*a2 - a1*a1 - a0*a3*a3;
@@ -156,31 +179,31 @@ template <typename R> __device__ __forceinline__ void quartic_extremes(R d4, R d
   int ncub;
   {
     const R ca2 = au2, ca1 = au1, ca0 = au0;
-    const R q = ca1/R(3.0) - ca2*ca2/R(9.0);
-    const R r = (ca1*ca2 - R(3.0)*ca0)/R(6.0) - ca2*ca2*ca2 / R(27.0);
+    const R q = ca1*R(1.0/3.0) - ca2*ca2*R(1.0/9.0);
+    const R r = (ca1*ca2 - R(3.0)*ca0)*R(1.0/6.0) - ca2*ca2*ca2*R(1.0/27.0);
     const R delta = q*q*q + r*r;
     if (delta > R(0)) {
       R s1 = r + dsqrt(delta);
       s1 = (s1 >= R(0)) ? dcbrt_pow(s1) : -dcbrt_pow(-s1);
       R s2 = r - dsqrt(delta);
       s2 = (s2 >= R(0)) ? dcbrt_pow(s2) : -dcbrt_pow(-s2);
-      x1 = (s1+s2) - ca2/R(3.0);
-      x3 = R(-0.5) * (s1+s2) - ca2/R(3.0);
+      x1 = (s1+s2) - ca2*R(1.0/3.0);
+      x3 = R(-0.5) * (s1+s2) - ca2*R(1.0/3.0);
       ncub = 1;
     }
     else if (delta < R(0)) {
-      const R theta = dacos(r/dsqrt(-q*q*q)) / R(3.0);
+      const R theta = dacos(r*rcp(dsqrt(-q*q*q))) * R(1.0/3.0);
       const R costh = dcos(theta);
       const R sinth = dsin(theta);
       const R sq = dsqrt(-q);
-      x1 = R(2.0)*sq*costh - ca2/R(3.0);
-      x3 = -sq*costh - ca2/R(3.0) + dsqrt(R(3.)) * sq * sinth;
+      x1 = R(2.0)*sq*costh - ca2*R(1.0/3.0);
+      x3 = -sq*costh - ca2*R(1.0/3.0) + dsqrt(R(3.)) * sq * sinth;
       ncub = 3;
     }
     else {
       const R s = (r >= R(0)) ? dcbrt_pow(r) : -dcbrt_pow(-r);
-      x1 = R(2.0)*s - ca2/R(3.0);
-      x3 = -s - ca2/R(3.0);
+      x1 = R(2.0)*s - ca2*R(1.0/3.0);
+      x3 = -s - ca2*R(1.0/3.0);
       ncub = 3;
     }
   }
@@ -190,7 +213,7 @@ template <typename R> __device__ __forceinline__ void quartic_extremes(R d4, R d
   R D2, E2;
   if (Rr != R(0)) {
     const R foo1 = R(0.75)*a3*a3 - R2 - R(2.0)*a2;
-    const R foo2 = R(0.25)*(R(4.0)*a3*a2 - R(8.0)*a1 - a3*a3*a3) / Rr;
+    const R foo2 = R(0.25)*(R(4.0)*a3*a2 - R(8.0)*a1 - a3*a3*a3) * rcp(Rr);
     D2 = foo1 + foo2;
     E2 = foo1 - foo2;
   }
@@ -235,26 +258,28 @@ __device__ __forceinline__ void flux_and_eigenvalues(const R *U, const R *P, R *
   const R V2 = P[vx]*P[vx] + P[vy]*P[vy] + P[vz]*P[vz];
   const R B2 = P[Bx]*P[Bx] + P[By]*P[By] + P[Bz]*P[Bz];
   const R Bv = P[Bx]*P[vx] + P[By]*P[vy] + P[Bz]*P[vz];
-  const R W2 = R(1) / (R(1) - V2);
+  const R W2 = rcp(R(1) - V2);
   const R W = dsqrt(W2);
   const R b0 = W*Bv;
-  const R b2 = (B2 + b0*b0) / W2;
-  const R bx = (P[Bx] + b0 * W*P[vx]) / W;
-  const R by = (P[By] + b0 * W*P[vy]) / W;
-  const R bz = (P[Bz] + b0 * W*P[vz]) / W;
-  const R e = (P[pre] / gm1) / P[rho];
+  const R b2 = (B2 + b0*b0) * (R(1) - V2);
+  const R iW = rcp(W), irho = rcp(P[rho]), igm1 = rcp(gm1);
+  const R bx = (P[Bx] + b0 * W*P[vx]) * iW;
+  const R by = (P[By] + b0 * W*P[vy]) * iW;
+  const R bz = (P[Bz] + b0 * W*P[vz]) * iW;
+  const R e = (P[pre] * igm1) * irho;
   const R p_ = P[pre] + R(0.5) * b2;
-  const R h = R(1) + e + P[pre]/P[rho];
+  const R h = R(1) + e + P[pre]*irho;
 
   constexpr int VN = DIM == 1 ? vx : DIM == 2 ? vy : vz;
   constexpr int BN = DIM == 1 ? Bx : DIM == 2 ? By : Bz;
   const R vn = P[VN];
   const R Bn = P[BN];
   F[ddd] = U[ddd] * vn;
-  F[tau] = U[tau] * vn - b0*Bn / W + p_*vn;
-  F[Sx] = U[Sx] * vn - bx*Bn / W;
-  F[Sy] = U[Sy] * vn - by*Bn / W;
-  F[Sz] = U[Sz] * vn - bz*Bn / W;
+  const R BniW = Bn * iW;
+  F[tau] = U[tau] * vn - b0*BniW + p_*vn;
+  F[Sx] = U[Sx] * vn - bx*BniW;
+  F[Sy] = U[Sy] * vn - by*BniW;
+  F[Sz] = U[Sz] * vn - bz*BniW;
   if (DIM == 1) F[Sx] = F[Sx] + p_;
   if (DIM == 2) F[Sy] = F[Sy] + p_;
   if (DIM == 3) F[Sz] = F[Sz] + p_;
@@ -265,14 +290,14 @@ __device__ __forceinline__ void flux_and_eigenvalues(const R *U, const R *P, R *
 
   const R vi = vn;
   const R bi = DIM == 1 ? bx : DIM == 2 ? by : bz;
-  const R rhoh = P[rho] + P[pre] / gm1 + P[pre];
-  const R cs2 = gamma*P[pre] / rhoh;
+  const R rhoh = P[rho] + P[pre] * igm1 + P[pre];
   const R W4 = W2*W2;
   const R v2 = vi*vi;
   const R v3 = vi*v2;
   const R v4 = vi*v3;
-  const R K = P[rho]*h * (R(1.0)/cs2 - R(1)) * W4;
-  const R L = -(P[rho]*h + b2/cs2) * W2;
+  const R ics2 = rhoh * rcp(gamma*P[pre]);
+  const R K = P[rho]*h * (ics2 - R(1)) * W4;
+  const R L = -(P[rho]*h + b2*ics2) * W2;
   const R A4 = K - L - b0*b0;
   const R A3 = R(-4)*K*vi + L*vi*R(2) + R(2)*b0*bi;
   const R A2 = R(6)*K*v2 + L*(R(1.0)-v2) + b0*b0 - bi*bi;
@@ -286,6 +311,31 @@ __device__ __forceinline__ void flux_and_eigenvalues(const R *U, const R *P, R *
   if (dfabs(ap) > R(1.0) || dfabs(am) > R(1.0)) {
     ap = R(1.0);
     am = R(-1.0);
+  }
+}
+
+/* Fluxes and fast speeds of both sides of an interface; one copy of the code runs
+ * twice (the quartic is the bulkiest piece of a Riemann solve). */
+template <typename R, int DIM>
+__device__ __forceinline__ void flux_eig_pair(const R *Pl, const R *Pr, const R *Ul, const R *Ur,
+                                              R *Fl, R *Fr, R &epl, R &eml, R &epr, R &emr, R gamma)
+{
+#pragma unroll Shape<R>::unroll
+  for (int side = 0; side < 2; ++side) {
+    R P[8], U[8], F[8], ep, em;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      P[i] = side ? Pr[i] : Pl[i];
+      U[i] = side ? Ur[i] : Ul[i];
+    }
+    flux_and_eigenvalues<R, DIM>(U, P, F, ep, em, gamma);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (side) Fr[i] = F[i];
+      else Fl[i] = F[i];
+    }
+    if (side) { epr = ep; emr = em; }
+    else { epl = ep; eml = em; }
   }
 }
 
@@ -316,7 +366,7 @@ __device__ __forceinline__ void c2p_new_state(C2PState &s, const double *U, doub
 __device__ __forceinline__ void c2p_estimate_from_cons(const C2PState &s, double &Z, double &W)
 {
   Z = sqrt(s.S2 + s.D*s.D);   // rmhd-c2p.c:97-101
-  W = Z / s.D;
+  W = Z * rcp(s.D);
 }
 
 __device__ __forceinline__ void c2p_starting_prim(const C2PState &s, const double *P, double &Z, double &W)
@@ -334,12 +384,14 @@ __device__ __forceinline__ int c2p_reconstruct(const C2PState &s, double Z, doub
 {
   // rmhd-c2p.c:121-150; Pout is written only on success
   double P[8];
-  const double b0 = s.BS * W / Z;
-  P[rho] = s.D/W;
-  P[pre] = (s.D/W) * (Z/(s.D*W) - 1.0) * s.gamf;
-  P[vx] = (s.Cons[Sx] + b0*s.Cons[Bx]/W) / (Z+s.B2);
-  P[vy] = (s.Cons[Sy] + b0*s.Cons[By]/W) / (Z+s.B2);
-  P[vz] = (s.Cons[Sz] + b0*s.Cons[Bz]/W) / (Z+s.B2);
+  const double iW = rcp(W), iZB = rcp(Z + s.B2);
+  const double b0 = s.BS * W * rcp(Z);
+  const double b0iW = b0 * iW;
+  P[rho] = s.D*iW;
+  P[pre] = (s.D*iW) * (Z*iW*rcp(s.D) - 1.0) * s.gamf;
+  P[vx] = (s.Cons[Sx] + b0iW*s.Cons[Bx]) * iZB;
+  P[vy] = (s.Cons[Sy] + b0iW*s.Cons[By]) * iZB;
+  P[vz] = (s.Cons[Sz] + b0iW*s.Cons[Bz]) * iZB;
   P[Bx] = s.Cons[Bx];
   P[By] = s.Cons[By];
   P[Bz] = s.Cons[Bz];
@@ -351,26 +403,30 @@ __device__ __forceinline__ int c2p_reconstruct(const C2PState &s, double Z, doub
   return C2P_SUCCESS;
 }
 
-/* rmhd_c2p_solve_anton2dzw, Mara/rmhd-c2p.c:182-253 (Anton et al. 2006, eqs. 84-85) */
+/* rmhd_c2p_solve_anton2dzw, Mara/rmhd-c2p.c:182-253 (Anton et al. 2006, eqs. 84-85).
+ * The residuals and Jacobian are the reference's; the powers of 1/Z and 1/W are formed
+ * from one reciprocal each per iteration (the reference divides a dozen times). */
 __device__ __forceinline__ int c2p_anton2dzw(const C2PState &s, double Z, double W, double *P, int &iters)
 {
   double error = 1.0;
   int it = 0;
   const double D = s.D, B2 = s.B2, BS2 = s.BS2, S2 = s.S2, Tau = s.Tau, gamf = s.gamf;
+  const double iD = rcp(D);
+  double iZ = rcp(Z), iW = rcp(W);
   while (error > 1e-12) {
     const double Z2 = Z*Z;
     const double Z3 = Z*Z2;
     const double W2 = W*W;
-    const double W3 = W*W2;
-    const double Pre = (D/W) * (Z/(D*W) - 1.0) * gamf;
-    const double df0dZ = 2*(B2+Z)*(BS2*W2 + (W2-1)*Z3) / (W2*Z3);
-    const double df0dW = 2*(B2+Z)*(B2+Z) / W3;
-    const double df1dZ = 1.0 + BS2/Z3 - gamf/W2;
-    const double df1dW = B2/W3 + (2*Z - D*W)/W3 * gamf;
-    const double f0 = -S2 + (Z+B2)*(Z+B2)*(W2-1)/W2 - (2*Z+B2)*BS2/Z2;
-    const double f1 = -Tau + Z+B2 - Pre - 0.5*B2/W2 - 0.5*BS2/Z2 - D;
-    const double det = df0dZ*df1dW - df0dW*df1dZ;
-    const double G0 = df1dW/det, G1 = -df0dW/det, G2 = -df1dZ/det, G3 = df0dZ/det;
+    const double iZ2 = iZ*iZ, iZ3 = iZ2*iZ, iW2 = iW*iW, iW3 = iW2*iW;
+    const double Pre = (D*iW) * (Z*iW*iD - 1.0) * gamf;
+    const double df0dZ = 2*(B2+Z)*(BS2*W2 + (W2-1)*Z3) * (iW2*iZ3);
+    const double df0dW = 2*(B2+Z)*(B2+Z) * iW3;
+    const double df1dZ = 1.0 + BS2*iZ3 - gamf*iW2;
+    const double df1dW = B2*iW3 + (2*Z - D*W)*iW3 * gamf;
+    const double f0 = -S2 + (Z+B2)*(Z+B2)*(W2-1)*iW2 - (2*Z+B2)*BS2*iZ2;
+    const double f1 = -Tau + Z+B2 - Pre - 0.5*B2*iW2 - 0.5*BS2*iZ2 - D;
+    const double idet = rcp(df0dZ*df1dW - df0dW*df1dZ);
+    const double G0 = df1dW*idet, G1 = -df0dW*idet, G2 = -df1dZ*idet, G3 = df0dZ*idet;
     const double dZ = -(G0*f0 + G1*f1);
     const double dW = -(G2*f0 + G3*f1);
     double Z_new = Z + dZ;
@@ -381,7 +437,9 @@ __device__ __forceinline__ int c2p_anton2dzw(const C2PState &s, double Z, double
     W_new = (W_new < 1e12) ? W_new : 1e12;
     Z = Z_new;
     W = W_new;
-    error = fabs(dZ/Z) + fabs(dW/W);
+    iZ = rcp(Z);
+    iW = rcp(W);
+    error = fabs(dZ*iZ) + fabs(dW*iW);
     ++it;
     if (it == 250) { iters += it; return C2P_MAXITER; }
   }
@@ -395,31 +453,36 @@ __device__ __forceinline__ int c2p_noble1dw(const C2PState &s, double Z, double 
   double error = 1.0;
   int it = 0;
   const double D = s.D, B2 = s.B2, BS2 = s.BS2, S2 = s.S2, Tau = s.Tau, gamf = s.gamf;
+  const double iD = rcp(D);
+  double iZ = rcp(Z);
   while (error > 1e-12) {
     const double Z2 = Z*Z;
-    const double Z3 = Z*Z2;
+    const double iZ2 = iZ*iZ, iZ3 = iZ2*iZ;
     const double a = S2*Z2 + BS2*(B2 + 2*Z);
     const double b = (B2 + Z)*(B2 + Z)*Z2;
     const double ap = 2*(S2*Z + BS2);
     const double bp = 2*Z*(B2 + Z)*(B2 + 2*Z);
-    const double V2 = a / b;
-    const double W2 = 1.0 / (1.0 - V2);
+    const double ib = rcp(b);
+    const double V2 = a * ib;
+    const double W2 = rcp(1.0 - V2);
     const double W = sqrt(W2);
     const double W3 = W*W2;
-    const double Pre = (D/W) * (Z/(D*W) - 1.0) * gamf;
-    const double dv2dZ = (ap*b - bp*a) / (b*b);
-    const double delPdelZ = gamf/W2;
-    const double delPdelW = gamf*(D/W2 - 2*Z/W3);
+    const double iW = rcp(W);
+    const double Pre = (D*iW) * (Z*iW*iD - 1.0) * gamf;
+    const double dv2dZ = (ap*b - bp*a) * (ib*ib);
+    const double delPdelZ = gamf*(1.0 - V2);
+    const double delPdelW = gamf*(D*(1.0 - V2) - 2*Z*(iW*iW*iW));
     const double dWdv2 = 0.5*W3;
     const double dPdZ = delPdelW * dWdv2 * dv2dZ + delPdelZ;
-    const double f = Tau + D - 0.5*B2*(1+V2) + 0.5*BS2/Z2 - Z + Pre;
-    const double g = -0.5*B2*dv2dZ - BS2/Z3 - 1.0 + dPdZ;
-    const double dZ = -f/g;
+    const double f = Tau + D - 0.5*B2*(1+V2) + 0.5*BS2*iZ2 - Z + Pre;
+    const double g = -0.5*B2*dv2dZ - BS2*iZ3 - 1.0 + dPdZ;
+    const double dZ = -f*rcp(g);
     double Z_new = Z + dZ;
     Z_new = (Z_new > 0.0) ? Z_new : -Z_new;
     Z_new = (Z_new < 1e20) ? Z_new : Z;
     Z = Z_new;
-    error = fabs(dZ/Z);
+    iZ = rcp(Z);
+    error = fabs(dZ*iZ);
     ++it;
     if (it == 250) { iters += it; return C2P_MAXITER; }
   }
@@ -427,8 +490,8 @@ __device__ __forceinline__ int c2p_noble1dw(const C2PState &s, double Z, double 
   const double Z2 = Z*Z;
   const double a = S2*Z2 + BS2*(B2 + 2*Z);
   const double b = (B2 + Z)*(B2 + Z)*Z2;
-  const double V2 = a / b;
-  const double W2 = 1.0 / (1.0 - V2);
+  const double V2 = a * rcp(b);
+  const double W2 = rcp(1.0 - V2);
   const double W = sqrt(W2);
   return c2p_reconstruct(s, Z, W, P);
 }
@@ -439,6 +502,7 @@ __device__ __forceinline__ int c2p_noble1dw(const C2PState &s, double Z, double 
  * Mara/mara.cpp:321), whose guess fails its first Newton step as NaN; those two
  * attempts are then skipped with the same outcome. `first` returns which attempt
  * succeeded (0..3) or 4. */
+template <int UNROLL = 1>
 __device__ __forceinline__ int cons_to_prim(const double *U, double *P, double gamma, double pfloor,
                                             bool prev_valid, int &iters, int &first)
 {
@@ -447,30 +511,27 @@ __device__ __forceinline__ int cons_to_prim(const double *U, double *P, double g
   if (error) return error;
   C2PState s;
   c2p_new_state(s, U, gamma, pfloor);
-  double Z, W;
-  c2p_estimate_from_cons(s, Z, W);
-  error = c2p_anton2dzw(s, Z, W, P, iters);
-  if (!error) { first = 0; return 0; }
-  double Zp = 0.0, Wp = 1.0;
-  if (prev_valid) {
-    c2p_starting_prim(s, P, Zp, Wp);
-    error = c2p_anton2dzw(s, Zp, Wp, P, iters);
-    if (!error) { first = 1; return 0; }
-  }
-  else {
-    iters += 1;
-    error = C2P_PRIM_CONTAINS_NAN;
-  }
-  error = c2p_noble1dw(s, Z, P, iters);
-  if (!error) { first = 2; return 0; }
-  if (prev_valid) {
-    c2p_starting_prim(s, P, Zp, Wp);
-    error = c2p_noble1dw(s, Zp, P, iters);
-    if (!error) { first = 3; return 0; }
-  }
-  else {
-    iters += 1;
-    error = C2P_PRIM_CONTAINS_NAN;
+  // attempts in the reference's order: anton2dzw from the estimate, then from the
+  // previous state, noble1dw from the estimate, then from the previous state
+#pragma unroll UNROLL
+  for (int solver = 0; solver < 2; ++solver) {
+#pragma unroll UNROLL
+    for (int guess = 0; guess < 2; ++guess) {
+      if (guess == 1 && !prev_valid) {
+        // the zero-filled state gives Z = NaN: one step, then a NaN primitive state
+        iters += 1;
+        error = C2P_PRIM_CONTAINS_NAN;
+        continue;
+      }
+      double Z, W;
+      if (guess == 0) c2p_estimate_from_cons(s, Z, W);
+      else c2p_starting_prim(s, P, Z, W);
+      error = solver == 0 ? c2p_anton2dzw(s, Z, W, P, iters) : c2p_noble1dw(s, Z, P, iters);
+      if (!error) {
+        first = 2 * solver + guess;
+        return 0;
+      }
+    }
   }
   return error;
 }
@@ -536,13 +597,21 @@ template <typename R, bool C2R> __device__ __forceinline__ R weno5(R vm2, R vm1,
     w2 = d2 / ((epsp + B2)*(epsp + B2));
   }
   else {
+    // Jiang & Shu (1996): w_k = d_k / (eps + B_k)^2. Multiplying all three by
+    // prod_j (eps + B_j)^2 leaves the normalised result unchanged and needs no division.
     const R epsp = R(1e-6);
-    w0 = d0 / ((epsp + B0)*(epsp + B0));
-    w1 = d1 / ((epsp + B1)*(epsp + B1));
-    w2 = d2 / ((epsp + B2)*(epsp + B2));
+    R a0 = (epsp + B0)*(epsp + B0), a1 = (epsp + B1)*(epsp + B1), a2 = (epsp + B2)*(epsp + B2);
+    if (sizeof(R) == 4) {
+      const R m01 = a0 > a1 ? a0 : a1;
+      const R im = rcp(m01 > a2 ? m01 : a2);
+      a0 *= im; a1 *= im; a2 *= im;
+    }
+    w0 = d0 * (a1*a2);
+    w1 = d1 * (a0*a2);
+    w2 = d2 * (a0*a1);
   }
   const R wtot = w0 + w1 + w2;
-  return (w0*vs0 + w1*vs1 + w2*vs2)/wtot;
+  return (w0*vs0 + w1*vs1 + w2*vs2) * rcp(wtot);
 }
 
 /* ---------------------------------------------------------------------------
@@ -563,8 +632,7 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hll(const 
   const R gm1 = gamma - R(1);
   if (prim_to_cons(Pl, Ul, gm1)) return 1;
   if (prim_to_cons(Pr, Ur, gm1)) return 1;
-  flux_and_eigenvalues<R, DIM>(Ul, Pl, Fl, epl, eml, gamma);
-  flux_and_eigenvalues<R, DIM>(Ur, Pr, Fr, epr, emr, gamma);
+  flux_eig_pair<R, DIM>(Pl, Pr, Ul, Ur, Fl, Fr, epl, eml, epr, emr, gamma);
   const R ap = (epl > epr) ? epl : epr;
   const R am = (eml < emr) ? eml : emr;
   note_lambda(lam, am, ap);
@@ -574,7 +642,7 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hll(const 
   }
   else if (R(0) <= ap) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) F[i] = (ap*Fl[i] - am*Fr[i] + ap*am*(Ur[i] - Ul[i])) / (ap - am);
+    for (int i = 0; i < 8; ++i) F[i] = (ap*Fl[i] - am*Fr[i] + ap*am*(Ur[i] - Ul[i])) * rcp(ap - am);
   }
   else {
 #pragma unroll
@@ -612,8 +680,7 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hllc(const
   if (prim_to_cons(Pl, Ul, gm1)) return 1;
   if (prim_to_cons(Pr, Ur, gm1)) return 1;
   R epl, epr, eml, emr;
-  flux_and_eigenvalues<R, DIM>(Ul, Pl, Fl, epl, eml, gamma);
-  flux_and_eigenvalues<R, DIM>(Ur, Pr, Fr, epr, emr, gamma);
+  flux_eig_pair<R, DIM>(Pl, Pr, Ul, Ur, Fl, Fr, epl, eml, epr, emr, gamma);
   Ul[tau] += Ul[ddd]; Fl[tau] += Fl[ddd];
   Ur[tau] += Ur[ddd]; Fr[tau] += Fr[ddd];
   const R ap = (epl > epr) ? epl : epr;
@@ -627,10 +694,11 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hllc(const
     return 0;
   }
   R U_hll[8], F_hll[8];
+  const R ida = rcp(ap - am);
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    U_hll[i] = (ap*Ur[i] - am*Ul[i] + (Fl[i] - Fr[i])) / (ap - am);
-    F_hll[i] = (ap*Fl[i] - am*Fr[i] + ap*am*(Ur[i] - Ul[i])) / (ap - am);
+    U_hll[i] = (ap*Ur[i] - am*Ul[i] + (Fl[i] - Fr[i])) * ida;
+    F_hll[i] = (ap*Fl[i] - am*Fr[i] + ap*am*(Ur[i] - Ul[i])) * ida;
   }
   R Ul_[8], Ur_[8], lc;
   if (dfabs(U_hll[B1]) > T::hllc_small_bx) {
@@ -710,7 +778,7 @@ template <typename R> struct Hlld {
   R LE, LD, LB0, LB1, LB2, Lm0, Lm1, Lm2;
   R RE, RD, RB0, RB1, RB2, Rm0, Rm1, Rm2;
 
-  struct Inner { R v0, v1, v2, B0, B1, B2; };
+  struct Inner { R v0, v1, v2, B0, B1, B2, id; };
 
   /* FillInnerWaveStates, :321-384 */
   __device__ __forceinline__ void fill(R p, R Kl[3], R Kr[3], R Bc[3], R &etaL, R &etaR,
@@ -724,12 +792,16 @@ template <typename R> struct Hlld {
       const R Q = -A - G + (UlBx*UlBx)*(R(1) - am*am);
       R X = UlBx*(A*am*UlBx + C) - (A + G)*(am*p + LE);
       if (dfabs(X) < T::X_floor) X = T::X_floor;
-      aL.v0 = (UlBx*(A*UlBx + am*C) - (A + G)*(p + Lm0)) / X;
-      aL.v1 = (Q*Lm1 + LB1*(C + UlBx*(am*Lm0 - LE))) / X;
-      aL.v2 = (Q*Lm2 + LB2*(C + UlBx*(am*Lm0 - LE))) / X;
-      aL.B0 = (LB0 - UlBx*aL.v0) / (am - aL.v0);
-      aL.B1 = (LB1 - UlBx*aL.v1) / (am - aL.v0);
-      aL.B2 = (LB2 - UlBx*aL.v2) / (am - aL.v0);
+      const R iX = rcp(X);
+      const R CB = C + UlBx*(am*Lm0 - LE);
+      aL.v0 = (UlBx*(A*UlBx + am*C) - (A + G)*(p + Lm0)) * iX;
+      aL.v1 = (Q*Lm1 + LB1*CB) * iX;
+      aL.v2 = (Q*Lm2 + LB2*CB) * iX;
+      const R id = rcp(am - aL.v0);
+      aL.B0 = (LB0 - UlBx*aL.v0) * id;
+      aL.B1 = (LB1 - UlBx*aL.v1) * id;
+      aL.B2 = (LB2 - UlBx*aL.v2) * id;
+      aL.id = id;
     }
     {
       const R A = Rm0 - ap*RE + p*(R(1) - ap*ap);
@@ -738,34 +810,39 @@ template <typename R> struct Hlld {
       const R Q = -A - G + (UrBx*UrBx)*(R(1) - ap*ap);
       R X = UrBx*(A*ap*UrBx + C) - (A + G)*(ap*p + RE);
       if (dfabs(X) < T::X_floor) X = T::X_floor;
-      aR.v0 = (UrBx*(A*UrBx + ap*C) - (A + G)*(p + Rm0)) / X;
-      aR.v1 = (Q*Rm1 + RB1*(C + UrBx*(ap*Rm0 - RE))) / X;
-      aR.v2 = (Q*Rm2 + RB2*(C + UrBx*(ap*Rm0 - RE))) / X;
-      aR.B0 = (RB0 - UrBx*aR.v0) / (ap - aR.v0);
-      aR.B1 = (RB1 - UrBx*aR.v1) / (ap - aR.v0);
-      aR.B2 = (RB2 - UrBx*aR.v2) / (ap - aR.v0);
+      const R iX = rcp(X);
+      const R CB = C + UrBx*(ap*Rm0 - RE);
+      aR.v0 = (UrBx*(A*UrBx + ap*C) - (A + G)*(p + Rm0)) * iX;
+      aR.v1 = (Q*Rm1 + RB1*CB) * iX;
+      aR.v2 = (Q*Rm2 + RB2*CB) * iX;
+      const R id = rcp(ap - aR.v0);
+      aR.B0 = (RB0 - UrBx*aR.v0) * id;
+      aR.B1 = (RB1 - UrBx*aR.v1) * id;
+      aR.B2 = (RB2 - UrBx*aR.v2) * id;
+      aR.id = id;
     }
-    wL = p + (LE - (aL.v0*Lm0 + aL.v1*Lm1 + aL.v2*Lm2)) / (am - aL.v0);
-    wR = p + (RE - (aR.v0*Rm0 + aR.v1*Rm1 + aR.v2*Rm2)) / (ap - aR.v0);
+    wL = p + (LE - (aL.v0*Lm0 + aL.v1*Lm1 + aL.v2*Lm2)) * aL.id;
+    wR = p + (RE - (aR.v0*Rm0 + aR.v1*Rm1 + aR.v2*Rm2)) * aR.id;
     if (wL < R(0)) wL *= R(-1);
     if (wR < R(0)) wR *= R(-1);
     etaL = R(-((UlBx > R(0)) - (UlBx < R(0)))) * dsqrt(wL);
     etaR = R(+((UrBx > R(0)) - (UrBx < R(0)))) * dsqrt(wR);
-    const R denL = am*p + LE + UlBx*etaL;
-    const R denR = ap*p + RE + UrBx*etaR;
-    Kl[0] = (Lm0 + p + LB0*etaL) / denL;
-    Kl[1] = (Lm1 + LB1*etaL) / denL;
-    Kl[2] = (Lm2 + LB2*etaL) / denL;
-    Kr[0] = (Rm0 + p + RB0*etaR) / denR;
-    Kr[1] = (Rm1 + RB1*etaR) / denR;
-    Kr[2] = (Rm2 + RB2*etaR) / denR;
+    const R idenL = rcp(am*p + LE + UlBx*etaL);
+    const R idenR = rcp(ap*p + RE + UrBx*etaR);
+    Kl[0] = (Lm0 + p + LB0*etaL) * idenL;
+    Kl[1] = (Lm1 + LB1*etaL) * idenL;
+    Kl[2] = (Lm2 + LB2*etaL) * idenL;
+    Kr[0] = (Rm0 + p + RB0*etaR) * idenR;
+    Kr[1] = (Rm1 + RB1*etaR) * idenR;
+    Kr[2] = (Rm2 + RB2*etaR) * idenR;
     const R dK = Kr[0] - Kl[0];
+    const R idK = rcp(dK);
     const bool zeroBc = dfabs(dK) < T::small_delK;
     const R aRv[3] = { aR.v0, aR.v1, aR.v2 }, aLv[3] = { aL.v0, aL.v1, aL.v2 };
     const R aRB[3] = { aR.B0, aR.B1, aR.B2 }, aLB[3] = { aL.B0, aL.B1, aL.B2 };
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      Bc[k] = ((aRB[k]*(Kr[0] - aR.v0) + aR.B0*aRv[k]) - (aLB[k]*(Kl[0] - aL.v0) + aL.B0*aLv[k])) / dK;
+      Bc[k] = ((aRB[k]*(Kr[0] - aR.v0) + aR.B0*aRv[k]) - (aLB[k]*(Kl[0] - aL.v0) + aL.B0*aLv[k])) * idK;
       if (zeroBc) Bc[k] = R(0);
     }
   }
@@ -780,10 +857,10 @@ template <typename R> struct Hlld {
     fill(p, Kl, Kr, Bc, etaL, etaR, wL, wR, aL, aR);
     const R delK = Kr[0] - Kl[0];
     const R Bch0 = Bc[0]*delK, Bch1 = Bc[1]*delK, Bch2 = Bc[2]*delK;
-    const R Yl = (R(1) - (Kl[0]*Kl[0] + Kl[1]*Kl[1] + Kl[2]*Kl[2])) /
-                 (etaL*delK - (Kl[0]*Bch0 + Kl[1]*Bch1 + Kl[2]*Bch2));
-    const R Yr = (R(1) - (Kr[0]*Kr[0] + Kr[1]*Kr[1] + Kr[2]*Kr[2])) /
-                 (etaR*delK - (Kr[0]*Bch0 + Kr[1]*Bch1 + Kr[2]*Bch2));
+    const R Yl = (R(1) - (Kl[0]*Kl[0] + Kl[1]*Kl[1] + Kl[2]*Kl[2])) *
+                 rcp(etaL*delK - (Kl[0]*Bch0 + Kl[1]*Bch1 + Kl[2]*Bch2));
+    const R Yr = (R(1) - (Kr[0]*Kr[0] + Kr[1]*Kr[1] + Kr[2]*Kr[2])) *
+                 rcp(etaR*delK - (Kr[0]*Bch0 + Kr[1]*Bch1 + Kr[2]*Bch2));
     if (dfabs(delK) < T::small_delK) y = R(0);
     else if (dfabs(Bc[0]) < T::small_Bc) y = delK;
     else y = delK*(R(1) - Bc[0]*(Yr - Yl));
@@ -799,15 +876,21 @@ template <typename R> struct Hlld {
     R eps = R(0), eps_last;
     do {
       eps_last = eps;
-      R y, f0, f1;
-      if (!function(x, y)) { iters += it; return false; }
+      R y = R(0), f0 = R(0), f1 = R(0);
       const R dp = T::fd_relative ? T::fd_dp * (dfabs(x) > R(1e-3) ? dfabs(x) : R(1e-3)) : T::fd_dp;
-      const R p0 = x - R(0.5)*dp;
-      const R p1 = x + R(0.5)*dp;
-      if (!function(p0, f0)) { iters += it; return false; }
-      if (!function(p1, f1)) { iters += it; return false; }
-      const R J = (f1 - f0)/dp;
-      const R G = R(1.0) / J;
+      bool ok = true;
+#pragma unroll Shape<R>::unroll
+      for (int k = 0; k < 3; ++k) {
+        const R pk = k == 0 ? x : (k == 1 ? x - R(0.5)*dp : x + R(0.5)*dp);
+        R fk;
+        if (!function(pk, fk)) { ok = false; break; }
+        if (k == 0) y = fk;
+        else if (k == 1) f0 = fk;
+        else f1 = fk;
+      }
+      if (!ok) { iters += it; return false; }
+      const R J = (f1 - f0)*rcp(dp);
+      const R G = rcp(J);
       const R dx = R(0) + G*y;
       x -= dx;
       eps = (y > R(0)) ? y : -y;
@@ -819,10 +902,11 @@ template <typename R> struct Hlld {
     return true;
   }
 
-  /* HlldEquation48::ReconstructSolution, :417-538, for s = 0: the flux only.
-   * Fl/Fr/Ul/Ur are in the total-energy convention (tau + D). */
-  __device__ __forceinline__ bool flux(R p, const R *Ul, const R *Ur, const R *Fl, const R *Fr,
-                                       R *F) const
+  /* HlldEquation48::ReconstructSolution, :417-538, for s = 0: the flux only. The
+   * outer states U and F of the side the flux is taken from are recomputed here from
+   * the primitive states (bitwise the values of the first evaluation), which keeps them
+   * out of the registers during the Newton iterations. */
+  __device__ __forceinline__ bool flux(R p, const R *Pl, const R *Pr, R gamma, R *F) const
   {
     using T = Tol<R>;
     R Kl[3], Kr[3], Bc[3], etaL, etaR, wL, wR;
@@ -833,11 +917,12 @@ template <typename R> struct Hlld {
     const R denomL = etaL - (Kl[0]*Bc[0] + Kl[1]*Bc[1] + Kl[2]*Bc[2]);
     const R denomR = etaR - (Kr[0]*Bc[0] + Kr[1]*Bc[1] + Kr[2]*Bc[2]);
     R cLv[3], cRv[3];
+    const R idL = rcp(denomL), idR = rcp(denomR);
     const R aLv[3] = { aL.v0, aL.v1, aL.v2 }, aRv[3] = { aR.v0, aR.v1, aR.v2 };
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      cLv[k] = Kl[k] - Bc[k]*(R(1) - dKl) / denomL;
-      cRv[k] = Kr[k] - Bc[k]*(R(1) - dKr) / denomR;
+      cLv[k] = Kl[k] - Bc[k]*(R(1) - dKl) * idL;
+      cRv[k] = Kr[k] - Bc[k]*(R(1) - dKr) * idR;
       if (dfabs(denomL) < T::small_denom_c) cLv[k] = aLv[k];
       if (dfabs(denomR) < T::small_denom_c) cRv[k] = aRv[k];
     }
@@ -852,39 +937,44 @@ template <typename R> struct Hlld {
     if (dfabs(cRv[0] - cLv[0]) > T::contact) return false;
 
     // first region of the reference's chain (:518-524) whose upper speed bounds s = 0
-    if (R(0) <= am) {
+    const int region = R(0) <= am ? 0 : R(0) <= lmaL ? 1 : R(0) <= lamC ? 2
+                     : R(0) <= lmaR ? 3 : R(0) <= ap ? 4 : 5;
+    const bool left = region <= 2;
+    R Uo[8], Fo[8];
+    {
+      R Po[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) F[i] = Fl[i];
+      for (int i = 0; i < 8; ++i) Po[i] = left ? Pl[i] : Pr[i];
+      prim_to_cons(Po, Uo, gamma - R(1));
+      R e1, e2;
+      flux_and_eigenvalues<R, 1, false>(Uo, Po, Fo, e1, e2, gamma);
+      Uo[tau] += Uo[ddd];
+      Fo[tau] += Fo[ddd];
     }
-    else if (R(0) <= lmaL) {
-      R aLU[8];
-      state_a(aL, p, am, LD, LE, Ul, aLU);
+    const R lam_o = left ? am : ap;
+    if (region == 0 || region == 5) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) F[i] = Fl[i] + am*(aLU[i] - Ul[i]);
-    }
-    else if (R(0) <= lamC) {
-      R aLU[8], cLU[8];
-      state_a(aL, p, am, LD, LE, Ul, aLU);
-      state_c(aLU, aL.v0, cLv, Bc, p, lmaL, lamC, cLU);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) F[i] = Fl[i] + am*(aLU[i] - Ul[i]) + lmaL*(cLU[i] - aLU[i]);
-    }
-    else if (R(0) <= lmaR) {
-      R aRU[8], cRU[8];
-      state_a(aR, p, ap, RD, RE, Ur, aRU);
-      state_c(aRU, aR.v0, cRv, Bc, p, lmaR, lamC, cRU);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) F[i] = Fr[i] + ap*(aRU[i] - Ur[i]) + lmaR*(cRU[i] - aRU[i]);
-    }
-    else if (R(0) <= ap) {
-      R aRU[8];
-      state_a(aR, p, ap, RD, RE, Ur, aRU);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) F[i] = Fr[i] + ap*(aRU[i] - Ur[i]);
+      for (int i = 0; i < 8; ++i) F[i] = Fo[i];
     }
     else {
+      Inner a;
+      a.v0 = left ? aL.v0 : aR.v0; a.v1 = left ? aL.v1 : aR.v1; a.v2 = left ? aL.v2 : aR.v2;
+      a.B0 = left ? aL.B0 : aR.B0; a.B1 = left ? aL.B1 : aR.B1; a.B2 = left ? aL.B2 : aR.B2;
+      a.id = left ? aL.id : aR.id;
+      R aU[8];
+      state_a(a, p, lam_o, left ? LD : RD, left ? LE : RE, Uo, aU);
+      if (region == 1 || region == 4) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) F[i] = Fr[i];
+        for (int i = 0; i < 8; ++i) F[i] = Fo[i] + lam_o*(aU[i] - Uo[i]);
+      }
+      else {
+        R cU[8];
+        const R lma = left ? lmaL : lmaR;
+        const R cv[3] = { left ? cLv[0] : cRv[0], left ? cLv[1] : cRv[1], left ? cLv[2] : cRv[2] };
+        state_c(aU, a.v0, cv, Bc, p, lma, lamC, cU);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) F[i] = Fo[i] + lam_o*(aU[i] - Uo[i]) + lma*(cU[i] - aU[i]);
+      }
     }
     F[tau] -= F[ddd];
     return true;
@@ -902,8 +992,9 @@ template <typename R> struct Hlld {
       return;
     }
     const R vB = a.v0*a.B0 + a.v1*a.B1 + a.v2*a.B2;
-    Ua[ddd] = Dj / denom;
-    Ua[tau] = (Ej + p*a.v0 - vB*a.B0) / denom;
+    const R idn = rcp(denom);
+    Ua[ddd] = Dj * idn;
+    Ua[tau] = (Ej + p*a.v0 - vB*a.B0) * idn;
     Ua[Sx] = (Ua[tau] + p)*a.v0 - vB*a.B0;
     Ua[Sy] = (Ua[tau] + p)*a.v1 - vB*a.B1;
     Ua[Sz] = (Ua[tau] + p)*a.v2 - vB*a.B2;
@@ -924,8 +1015,9 @@ template <typename R> struct Hlld {
       return;
     }
     const R vBc = cv[0]*Bc[0] + cv[1]*Bc[1] + cv[2]*Bc[2];
-    cU[ddd] = aU[ddd] * (lma - av0) / denom;
-    cU[tau] = (lma*aU[tau] - aU[Sx] + p*lamC - vBc*Bc[0]) / denom;
+    const R idn = rcp(denom);
+    cU[ddd] = aU[ddd] * (lma - av0) * idn;
+    cU[tau] = (lma*aU[tau] - aU[Sx] + p*lamC - vBc*Bc[0]) * idn;
     cU[Sx] = (cU[tau] + p)*cv[0] - vBc*Bc[0];
     cU[Sy] = (cU[tau] + p)*cv[1] - vBc*Bc[1];
     cU[Sz] = (cU[tau] + p)*cv[2] - vBc*Bc[2];
@@ -942,14 +1034,18 @@ struct FluxCounters {
   int c2p_iters;     // nested c2p Newton iterations
 };
 
-/* HlldRmhdRiemannSolver::IntercellFlux, Mara/riemann_hlld-rmhd.cpp:143-277, s = 0 */
+/* HlldRmhdRiemannSolver::IntercellFlux, Mara/riemann_hlld-rmhd.cpp:143-277, s = 0.
+ * Live-state economy: the jump states across the fast waves are formed before the
+ * nested c2p, after which the outer U and F are dead; only the primitive states are
+ * kept (to rebuild the flux side at the end and for the HLLC fallback, which averages
+ * the normal field exactly as here, riemann_hllc-rmhd.cpp:60). */
 template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f,
                                                                        R gamma, double pfloor, R &lam,
                                                                        FluxCounters &cnt)
 {
   using T = Tol<R>;
   using Pm = Perm<DIM>;
-  R Pl[8], Pr[8], Ul[8], Ur[8], Fl[8], Fr[8];
+  R Pl[8], Pr[8];
   Pl[pre] = pl[pre]; Pr[pre] = pr[pre];
   Pl[rho] = pl[rho]; Pr[rho] = pr[rho];
   Pl[vx] = pl[Pm::v1]; Pr[vx] = pr[Pm::v1];
@@ -959,37 +1055,34 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
   Pl[Bz] = pl[Pm::B3]; Pr[Bz] = pr[Pm::B3];
   Pl[Bx] = Pr[Bx] = R(0.5)*(pl[Pm::B1] + pr[Pm::B1]);
 
-  const R gm1 = gamma - R(1);
-  if (prim_to_cons(Pl, Ul, gm1)) return 1;
-  if (prim_to_cons(Pr, Ur, gm1)) return 1;
-  R epl, epr, eml, emr;
-  flux_and_eigenvalues<R, 1>(Ul, Pl, Fl, epl, eml, gamma);
-  flux_and_eigenvalues<R, 1>(Ur, Pr, Fr, epr, emr, gamma);
-  const R am = (eml < emr) ? eml : emr;
-  const R ap = (epl > epr) ? epl : epr;
-  note_lambda(lam, am, ap);
-
-  // HLL state, its primitive inversion (always FP64), and the two starting guesses
+  Hlld<R> e;
   double Uh[8], Ph[8];
   R Uhll_tau, Uhll_ddd, Uhll_Sx, Fhll_tau, Fhll_ddd, Fhll_Sx;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const R uh = (ap*Ur[i] - am*Ul[i] + (Fl[i] - Fr[i])) / (ap - am);
-    Uh[i] = (double)uh;
-    Ph[i] = (double)(R(0.5)*(Pl[i] + Pr[i]));
-  }
-  Uhll_tau = (R)Uh[tau]; Uhll_ddd = (R)Uh[ddd]; Uhll_Sx = (R)Uh[Sx];
-  Fhll_tau = (ap*Fl[tau] - am*Fr[tau] + ap*am*(Ur[tau] - Ul[tau])) / (ap - am);
-  Fhll_ddd = (ap*Fl[ddd] - am*Fr[ddd] + ap*am*(Ur[ddd] - Ul[ddd])) / (ap - am);
-  Fhll_Sx = (ap*Fl[Sx] - am*Fr[Sx] + ap*am*(Ur[Sx] - Ul[Sx])) / (ap - am);
-  int c2p_first;
-  const int hll_c2p_failed = cons_to_prim(Uh, Ph, (double)gamma, pfloor, true, cnt.c2p_iters, c2p_first);
+  {
+    R Ul[8], Ur[8], Fl[8], Fr[8];
+    const R gm1 = gamma - R(1);
+    if (prim_to_cons(Pl, Ul, gm1)) return 1;
+    if (prim_to_cons(Pr, Ur, gm1)) return 1;
+    R epl, epr, eml, emr;
+    flux_eig_pair<R, 1>(Pl, Pr, Ul, Ur, Fl, Fr, epl, eml, epr, emr, gamma);
+    const R am = (eml < emr) ? eml : emr;
+    const R ap = (epl > epr) ? epl : epr;
+    note_lambda(lam, am, ap);
 
-  bool success = false;
-  R Fx[8];
-  if (!hll_c2p_failed) {
-    Hlld<R> e;
-    // HlldEquation48 constructor, :292-317
+    // HLL state (its primitive inversion is always FP64) and the eqn (55) inputs
+    const R ida = rcp(ap - am);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const R uh = (ap*Ur[i] - am*Ul[i] + (Fl[i] - Fr[i])) * ida;
+      Uh[i] = (double)uh;
+      Ph[i] = (double)(R(0.5)*(Pl[i] + Pr[i]));
+    }
+    Uhll_tau = (R)Uh[tau]; Uhll_ddd = (R)Uh[ddd]; Uhll_Sx = (R)Uh[Sx];
+    Fhll_tau = (ap*Fl[tau] - am*Fr[tau] + ap*am*(Ur[tau] - Ul[tau])) * ida;
+    Fhll_ddd = (ap*Fl[ddd] - am*Fr[ddd] + ap*am*(Ur[ddd] - Ul[ddd])) * ida;
+    Fhll_Sx = (ap*Fl[Sx] - am*Fr[Sx] + ap*am*(Ur[Sx] - Ul[Sx])) * ida;
+
+    // HlldEquation48 constructor, :292-317: jumps across the fast waves, eqn (12)
     Ul[tau] += Ul[ddd]; Fl[tau] += Fl[ddd];
     Ur[tau] += Ur[ddd]; Fr[tau] += Fr[ddd];
     e.am = am; e.ap = ap;
@@ -1002,7 +1095,14 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
     e.Lm0 = am*Ul[Sx] - Fl[Sx]; e.Rm0 = ap*Ur[Sx] - Fr[Sx];
     e.Lm1 = am*Ul[Sy] - Fl[Sy]; e.Rm1 = ap*Ur[Sy] - Fr[Sy];
     e.Lm2 = am*Ul[Sz] - Fl[Sz]; e.Rm2 = ap*Ur[Sz] - Fr[Sz];
+  }
+  int c2p_first;
+  const int hll_c2p_failed = cons_to_prim<Shape<R>::unroll>(Uh, Ph, (double)gamma, pfloor, true,
+                                                              cnt.c2p_iters, c2p_first);
 
+  bool success = false;
+  R Fx[8];
+  if (!hll_c2p_failed) {
     // EstimateSolution(0): total pressure of the inverted HLL state, :543-555
     R est0;
     {
@@ -1017,6 +1117,7 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
       est0 = (R)(P[pre] + 0.5*b2);
     }
     const bool small_bn = dfabs(Pl[Bx]) < T::small_bn || dfabs(Pr[Bx]) < T::small_bn;
+#pragma unroll Shape<R>::unroll
     for (int attempt = 0; attempt < 3 && !success; ++attempt) {
       R p_star;
       if (small_bn || attempt == 1) {
@@ -1034,8 +1135,8 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
       else {
         p_star = attempt == 0 ? est0 : R(0);
       }
-      if (small_bn) success = e.flux(p_star, Ul, Ur, Fl, Fr, Fx);
-      else success = e.newton(p_star, cnt.newton_iters) && e.flux(p_star, Ul, Ur, Fl, Fr, Fx);
+      if (small_bn) success = e.flux(p_star, Pl, Pr, gamma, Fx);
+      else success = e.newton(p_star, cnt.newton_iters) && e.flux(p_star, Pl, Pr, gamma, Fx);
     }
     if (success) {
       ++cnt.hlld_ok;
@@ -1052,7 +1153,17 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
     }
   }
   ++cnt.hlld_fail;
-  return riemann_hllc<R, DIM>(pl, pr, f, gamma, lam);
+  // HLLC of the same (normal-field-averaged) states in the lab frame
+  R ql[8], qr[8];
+  ql[rho] = Pl[rho]; qr[rho] = Pr[rho];
+  ql[pre] = Pl[pre]; qr[pre] = Pr[pre];
+  ql[Pm::v1] = Pl[vx]; qr[Pm::v1] = Pr[vx];
+  ql[Pm::v2] = Pl[vy]; qr[Pm::v2] = Pr[vy];
+  ql[Pm::v3] = Pl[vz]; qr[Pm::v3] = Pr[vz];
+  ql[Pm::B1] = Pl[Bx]; qr[Pm::B1] = Pr[Bx];
+  ql[Pm::B2] = Pl[By]; qr[Pm::B2] = Pr[By];
+  ql[Pm::B3] = Pl[Bz]; qr[Pm::B3] = Pr[Bz];
+  return riemann_hllc<R, DIM>(ql, qr, f, gamma, lam);
 }
 
 }  // namespace maragpu

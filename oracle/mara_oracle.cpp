@@ -542,7 +542,9 @@ void oracle_riemann_batch(long n, int solver, const double *PL, const double *PR
  * spread evenly through the reference's sweep range, and the c2p cascade (zero
  * previous-prim guess, as in the first RK stage) over `nsample` zones.
  * out[0..2] = mean flops per interface for x, y, z; out[3] = mean c2p flops per zone;
- * out[4..6] = mean transcendental calls per interface; out[7] = c2p transcendentals. */
+ * out[4..6] = mean transcendental calls per interface; out[7] = c2p transcendentals;
+ * out[8..22] = per axis (add, mul, div, sqrt, transcendental) means; out[23..27] the same
+ * for the c2p (out must hold 28 doubles). */
 void oracle_count_flops(void *h, const double *P_synced, long nsample, double *out)
 {
   Grid &g = *static_cast<Grid *>(h);
@@ -580,6 +582,12 @@ void oracle_count_flops(void *h, const double *P_synced, long nsample, double *o
     }
     out[dim - 1] = (double)total.flops() / nsample;
     out[3 + dim] = (double)total.trans / nsample;
+    // per-category detail: out[8 + 5*(dim-1) + {0 add, 1 mul, 2 div, 3 sqrt, 4 trans}]
+    out[8 + 5 * (dim - 1) + 0] = (double)total.add / nsample;
+    out[8 + 5 * (dim - 1) + 1] = (double)total.mul / nsample;
+    out[8 + 5 * (dim - 1) + 2] = (double)total.div / nsample;
+    out[8 + 5 * (dim - 1) + 3] = (double)total.sqrt / nsample;
+    out[8 + 5 * (dim - 1) + 4] = (double)total.trans / nsample;
   }
   FlopTally total;
   const long nzone = g.T;
@@ -596,6 +604,11 @@ void oracle_count_flops(void *h, const double *P_synced, long nsample, double *o
   }
   out[3] = (double)total.flops() / nsample;
   out[7] = (double)total.trans / nsample;
+  out[23] = (double)total.add / nsample;
+  out[24] = (double)total.mul / nsample;
+  out[25] = (double)total.div / nsample;
+  out[26] = (double)total.sqrt / nsample;
+  out[27] = (double)total.trans / nsample;
 }
 
 }  // extern "C"

@@ -237,7 +237,7 @@ class Solver:
         return m.value, x.value, a.value
 
     def count_flops(self, P_synced, nsample=2000):
-        out = np.zeros(8)
+        out = np.zeros(28)
         P = np.ascontiguousarray(P_synced, dtype=np.float64)
         self.lib.lib.oracle_count_flops(self.h, _d(P), nsample, _d(out))
         return out
