@@ -181,7 +181,21 @@ def main():
     ap.add_argument("--forms", default="plain,mixed,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--model", default=os.path.join(ROOT, "ctf_amd", "models", "c2p_mlp.bin"))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (calibrates the "
+                         "HBM byte counters read by rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
                                    "of it (see DESIGN.md for how the update shards)",
