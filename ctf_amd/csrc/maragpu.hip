@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "rmhd_device.hpp"
+#include "mlp_device.hpp"
 #include "../../include/maragpu.h"
 
 using namespace maragpu;
@@ -236,6 +237,154 @@ __global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restric
   unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
                                        &st->stage_failed[pass] };
   block_add<4>(cnt, dst);
+}
+
+/* Newton start (Z, W) of the anton2dzw solve from the network's raw outputs: W = exp(y0)
+ * clamped to [1, 1e6], p = D exp(y1), rho = D / W, h = 1 + Gamma/(Gamma-1) p / rho,
+ * Z = rho h W^2. */
+__device__ __forceinline__ void learned_start(double D, float y0, float y1, const MlpDev &net, double gamma,
+                                              double &Zg, double &Wg)
+{
+  double W = exp((double)(y0 * net.ystd[0] + net.ymean[0]));
+  W = W > 1.0 ? (W < 1e6 ? W : 1e6) : 1.0;
+  const double pg = D * exp((double)(y1 * net.ystd[1] + net.ymean[1]));
+  const double rg = D / W;
+  const double hg = 1.0 + gamma / (gamma - 1.0) * pg / rg;
+  Zg = rg * hg * W * W;
+  Wg = W;
+}
+
+/* Per-zone conserved->primitive with the learned initial guess: the network (one wave
+ * per 64 zones on the matrix cores) predicts log W and log(p/D); anton2dzw polishes
+ * from (Z, W) of that guess in FP64; a zone whose polish fails runs the plain cascade
+ * of k_cons_to_prim. Blocks loop over 256-zone chunks so the weights are staged into
+ * LDS once per block. */
+__global__ __launch_bounds__(kBlock) void k_cons_to_prim_learned(const double *__restrict__ U,
+                                                                 double *__restrict__ P, GridDesc g,
+                                                                 double gamma, double pfloor, int first,
+                                                                 int ereset, double preset, double dreset,
+                                                                 DevStats *st, int pass, MlpDev net)
+{
+  __shared__ uint16_t wf[MLP_NFWD * MLP_FRAG];
+  __shared__ float bias[3 * 64 + 2];
+  __shared__ uint16_t xin[kBlock / 64][64 * MLP_KIN];
+  __shared__ float yout[kBlock / 64][128];
+  for (int i = threadIdx.x; i < MLP_NFWD * MLP_FRAG / 4; i += kBlock)
+    reinterpret_cast<uint2 *>(wf)[i] = reinterpret_cast<const uint2 *>(net.fwd)[i];
+  for (int i = threadIdx.x; i < 3 * 64 + 2; i += kBlock)
+    bias[i] = i < 64 ? net.b1[i] : i < 128 ? net.b2[i - 64] : i < 192 ? net.b3[i - 128] : net.b4[i - 192];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  const long T = g.T;
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  for (long base = (long)blockIdx.x * kBlock; base < ni; base += (long)gridDim.x * kBlock) {
+    const long t = base + threadIdx.x;
+    const bool valid = t < ni;
+    double u[8], p[8];
+    int i = 0, j = 0, k = 0;
+    if (valid) {
+      k = (int)(t % g.N[2]) + g.Ng;
+      j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+      i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+      const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        u[q] = U[q * T + z];
+        p[q] = first ? 0.0 : P[q * T + z];
+      }
+    }
+    else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) { u[q] = q == 0 ? 1.0 : (q == 1 ? 1.0 : 0.0); p[q] = 0.0; }
+    }
+    float x[MLP_IN];
+    mlp_features(u, gamma, x);
+#pragma unroll
+    for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) / net.xstd[q];
+    mlp_input_row(x, &xin[wave][lane * MLP_KIN]);
+    __syncthreads();
+    mlp_forward_wave(xin[wave], wf, bias, bias + 64, bias + 128, bias + 192, yout[wave], nullptr);
+    __syncthreads();
+    const float y0 = yout[wave][lane * 2 + 0], y1 = yout[wave][lane * 2 + 1];
+    __syncthreads();
+    if (valid) {
+      int iters = 0, which = 0, used = 0;
+      int err = check_cons(u);
+      if (!err) {
+        C2PState s;
+        c2p_new_state(s, u, gamma, pfloor);
+        double Zg, Wg;
+        learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
+        err = c2p_anton2dzw(s, Zg, Wg, p, iters);
+        used = err == 0;
+        if (err) err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
+      }
+      int failed = err != 0;
+      if (failed && ereset) {
+        p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
+        p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
+        failed = 0;
+      }
+      const int mult = scatter_images(g, P, T, i, j, k, p);
+      cnt[0] += 1;
+      cnt[1] += (unsigned long long)iters;
+      cnt[2] += used ? 0ull : 1ull;
+      cnt[3] += failed ? (unsigned long long)mult : 0ull;
+      cnt[4] += 1;
+      cnt[5] += used ? 0ull : 1ull;
+    }
+  }
+  unsigned long long *const dst[6] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                       &st->stage_failed[pass], &st->mlp_guesses, &st->mlp_fallbacks };
+  block_add<6>(cnt, dst);
+}
+
+__global__ __launch_bounds__(kBlock) void kb_cons_to_prim_learned(long n, const double *__restrict__ U,
+                                                                  double *__restrict__ P, double gamma,
+                                                                  double pfloor, int *err_out, int *iters_out,
+                                                                  int *used_out, double *start_out, MlpDev net)
+{
+  __shared__ uint16_t xin[kBlock / 64][64 * MLP_KIN];
+  __shared__ float yout[kBlock / 64][128];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = t < n;
+  double u[8], p[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    u[q] = valid ? U[8 * t + q] : (q < 2 ? 1.0 : 0.0);
+    p[q] = valid ? P[8 * t + q] : 0.0;
+  }
+  float x[MLP_IN];
+  mlp_features(u, gamma, x);
+#pragma unroll
+  for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) / net.xstd[q];
+  mlp_input_row(x, &xin[wave][lane * MLP_KIN]);
+  __syncthreads();
+  mlp_forward_wave(xin[wave], net.fwd, net.b1, net.b2, net.b3, net.b4, yout[wave], nullptr);
+  __syncthreads();
+  if (!valid) return;
+  const float y0 = yout[wave][lane * 2 + 0], y1 = yout[wave][lane * 2 + 1];
+  int iters = 0, which = 0, used = 0;
+  double Zg = 0.0, Wg = 0.0;
+  int err = check_cons(u);
+  if (!err) {
+    C2PState s;
+    c2p_new_state(s, u, gamma, pfloor);
+    learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
+    err = c2p_anton2dzw(s, Zg, Wg, p, iters);
+    used = err == 0;
+    if (err) err = cons_to_prim(u, p, gamma, pfloor, true, iters, which);
+  }
+  if (start_out) {
+    start_out[2 * t + 0] = Zg;
+    start_out[2 * t + 1] = Wg;
+  }
+  err_out[t] = err;
+  iters_out[t] = iters;
+  used_out[t] = used;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) P[8 * t + q] = p[q];
 }
 
 /* --------------------------------------------------------------------------
@@ -545,8 +694,8 @@ struct maragpu_ctx {
   std::vector<hipEvent_t> ev;
   double flux_ms = 0.0, c2p_ms = 0.0, upd_ms = 0.0;
   int flux_n = 0, c2p_n = 0, upd_n = 0;
-  // learned c2p weights (device), see mlp.hip
-  void *mlp = nullptr;
+  // learned c2p network (not owned; see maragpu_attach_mlp)
+  const maragpu_mlp *mlp = nullptr;
 };
 
 namespace {
@@ -610,7 +759,7 @@ struct EventPair { int a, b; };
 
 }  // namespace
 
-extern "C" int maragpu_mlp_c2p_launch(maragpu_ctx *ctx, int first, int pass);
+extern "C" int maragpu_mlp_device_params(const maragpu_mlp *n, void *out_params);
 
 extern "C" {
 
@@ -670,7 +819,6 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   return MARAGPU_OK;
 }
 
-extern void maragpu_mlp_free(void *);
 
 int maragpu_destroy(maragpu_ctx *c)
 {
@@ -682,7 +830,6 @@ int maragpu_destroy(maragpu_ctx *c)
   if (c->hstats) hipHostFree(c->hstats);
   hipFree(c->ddiv);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
-  if (c->mlp) maragpu_mlp_free(c->mlp);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return MARAGPU_OK;
@@ -743,8 +890,12 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     EventPair p = mark(ec);
     HIP_CHECK(hipEventRecord(c->ev[p.a], c->stream));
     if (c->cfg.c2p == MARAGPU_C2P_LEARNED) {
-      const int rc = maragpu_mlp_c2p_launch(c, first, pass);
-      if (rc) return rc;
+      MlpDev net;
+      if (maragpu_mlp_device_params(c->mlp, &net)) return MARAGPU_E_NOMODEL;
+      const unsigned nb = nblocks(ni) < 4096u ? nblocks(ni) : 4096u;
+      k_cons_to_prim_learned<<<nb, kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+                                                           first, c->cfg.ereset, c->cfg.preset,
+                                                           c->cfg.dreset, c->dstats, pass, net);
     }
     else {
       k_cons_to_prim<<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
@@ -900,6 +1051,27 @@ int maragpu_kernel_times(maragpu_ctx *c, double *flux_ms, int *flux_n, double *c
   return MARAGPU_OK;
 }
 
+int maragpu_attach_mlp(maragpu_ctx *c, const maragpu_mlp *net)
+{
+  if (!c) return MARAGPU_E_INVALID;
+  c->mlp = net;
+  return MARAGPU_OK;
+}
+
+/* Committed primitive state on the device (SoA planes of T doubles) and the grid shape;
+ * used to sample training pairs from a running simulation. */
+const double *maragpu_committed_prim(maragpu_ctx *c, int *N, int *Ng, long *T, int *n1, int *n2)
+{
+  if (!c || !c->have_state) return nullptr;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+  for (int d = 0; d < 3; ++d) N[d] = c->g.N[d];
+  *Ng = c->g.Ng;
+  *T = c->g.T;
+  *n1 = c->g.n[1];
+  *n2 = c->g.n[2];
+  return c->Pc;
+}
+
 }  // extern "C"
 
 /* ---------------------------------------------------------------------------
@@ -978,6 +1150,32 @@ int maragpu_batch_reconstruct(size_t n, const double *V, double *out, int op, do
   kb_reconstruct<<<nblocks(n), kBlock>>>((long)n, dV.get<double>(), dO.get<double>(), op, theta, IS, sz10A);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpy(out, dO.p, n * 8, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+int maragpu_batch_cons_to_prim_learned(const maragpu_mlp *netp, size_t n, const double *U, double *P,
+                                       double gamma, double pfloor, int *err, int *iters, int *used,
+                                       double *start)
+{
+  if (!netp || !U || !P || !err || !iters || !used) return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  MlpDev net;
+  if (maragpu_mlp_device_params(netp, &net)) return MARAGPU_E_NOMODEL;
+  DevBuf dP, dU, dE, dI, dD, dS;
+  if (dalloc(dP, n * 64) || dalloc(dU, n * 64) || dalloc(dE, n * 4) || dalloc(dI, n * 4) || dalloc(dD, n * 4) ||
+      (start && dalloc(dS, n * 16)))
+    return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dP.p, P, n * 64, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(dU.p, U, n * 64, hipMemcpyHostToDevice));
+  kb_cons_to_prim_learned<<<nblocks(n), kBlock>>>((long)n, dU.get<double>(), dP.get<double>(), gamma, pfloor,
+                                                  dE.get<int>(), dI.get<int>(), dD.get<int>(),
+                                                  start ? dS.get<double>() : nullptr, net);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(P, dP.p, n * 64, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(err, dE.p, n * 4, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(iters, dI.p, n * 4, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(used, dD.p, n * 4, hipMemcpyDeviceToHost));
+  if (start) HIP_CHECK(hipMemcpy(start, dS.p, n * 16, hipMemcpyDeviceToHost));
   return MARAGPU_OK;
 }
 

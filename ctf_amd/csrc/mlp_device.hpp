@@ -1,0 +1,226 @@
+/* Learned conserved-to-primitive initial guess: a 6 -> 64 -> 64 -> 64 -> 2 MLP (SiLU)
+ * evaluated with BF16 matrix cores (v_mfma_f32_16x16x16_bf16) and FP32 accumulation.
+ *
+ * Layout idea. Each wave evaluates 64 samples. Every layer is computed transposed,
+ *     Z^T (out x samples) = W^T (out x in) . H^T (in x samples),
+ * so that the MFMA result layout (lane l holds rows 4*(l/16)+r, column l%16 of a 16x16
+ * tile) is exactly the B-operand layout the next layer needs (lane l holds k = 4*(l/16)+r
+ * of column l%16): activations stay in registers from layer to layer. Weights are stored
+ * pre-swizzled in "fragment order" (per 16x16 tile, 64 lanes x 4 bf16), so a lane loads
+ * its A operand with one 8-byte read.
+ *
+ * Features of a conserved state U = (D, tau, S, B) (all scale-free; D normalises):
+ *   x0 = log(tau/D), x1 = log(1 + |S|^2/D^2), x2 = log(1 + |B|^2/D),
+ *   x3 = S.B / (|S||B| + tiny), x4 = -log(1 - |S|^2/E^2) with E = tau + D, x5 = Gamma.
+ * x4 is -2 log W for cold unmagnetised flow; it is formed in FP64 as
+ * log(E^2) - log((E - |S|)(E + |S|)), free of the cancellation in 1 - |S|/E that would
+ * otherwise hide W > 10 below the inputs' precision. (|S.B| is x3 |S| |B| and needs no
+ * feature of its own.)
+ * standardised with the training set's mean and standard deviation. Each standardised
+ * feature enters the first layer as two bfloat16 columns, hi = bf16(x) and
+ * lo = bf16(x - hi), that share one weight column: the first product then sees x to about
+ * 16 mantissa bits instead of 8. Outputs (after de-standardising): y0 = log W,
+ * y1 = log(p/D).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace maragpu {
+
+constexpr int MLP_IN = 6;       // features
+constexpr int MLP_KIN = 16;     // padded input width (one MFMA k-tile)
+constexpr int MLP_H = 64;       // hidden width
+constexpr int MLP_OUT = 2;      // outputs
+constexpr int MLP_MOUT = 16;    // padded output width (one MFMA m-tile)
+constexpr int MLP_FRAG = 256;   // bf16 per 16x16 fragment (64 lanes x 4)
+
+// forward (A = W^T) fragments: layer 1: 4 m-tiles x 1 k-tile; layers 2, 3: 4 x 4;
+// layer 4: 1 x 4
+constexpr int MLP_F1 = 0, MLP_F2 = 4, MLP_F3 = 20, MLP_F4 = 36, MLP_NFWD = 40;
+// backward (A = W) fragments for the hidden-state gradients: layer 4: 4 m-tiles (in) x
+// 1 k-tile (out); layers 3, 2: 4 x 4
+constexpr int MLP_B4 = 0, MLP_B3 = 4, MLP_B2 = 20, MLP_NBWD = 36;
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+/* Parameters on the device. */
+struct MlpDev {
+  const uint16_t *fwd;      // MLP_NFWD fragments (bf16 bits)
+  const uint16_t *bwd;      // MLP_NBWD fragments
+  const float *b1, *b2, *b3, *b4;  // biases (64, 64, 64, 2)
+  float xmean[MLP_IN], xstd[MLP_IN];
+  float ymean[MLP_OUT], ystd[MLP_OUT];
+};
+
+__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+{
+  // round to nearest even (finite inputs; NaN stays NaN)
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+
+__device__ __forceinline__ float silu(float a) { return a / (1.0f + __expf(-a)); }
+__device__ __forceinline__ float silu_grad(float a)
+{
+  const float s = 1.0f / (1.0f + __expf(-a));
+  return s * (1.0f + a * (1.0f - s));
+}
+
+/* The first layer's input row of one sample: standardised features xs[MLP_IN] as
+ * hi/lo bfloat16 pairs in columns k and MLP_IN + k, zero padding after 2 MLP_IN. */
+__device__ __forceinline__ void mlp_input_row(const float *xs, uint16_t *row)
+{
+#pragma unroll
+  for (int k = 0; k < MLP_IN; ++k) {
+    const uint16_t hi = f32_to_bf16(xs[k]);
+    row[k] = hi;
+    row[MLP_IN + k] = f32_to_bf16(xs[k] - bf16_to_f32(hi));
+  }
+#pragma unroll
+  for (int k = 2 * MLP_IN; k < MLP_KIN; ++k) row[k] = 0;
+}
+
+__device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x4_t load_frag(const uint16_t *frags, int f, int lane)
+{
+  const uint2 v = *reinterpret_cast<const uint2 *>(frags + (size_t)f * MLP_FRAG + lane * 4);
+  bf16x4_t r;
+  r[0] = (short)(v.x & 0xffff);
+  r[1] = (short)(v.x >> 16);
+  r[2] = (short)(v.y & 0xffff);
+  r[3] = (short)(v.y >> 16);
+  return r;
+}
+
+__device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d)
+{
+  bf16x4_t r;
+  r[0] = (short)f32_to_bf16(a);
+  r[1] = (short)f32_to_bf16(b);
+  r[2] = (short)f32_to_bf16(c);
+  r[3] = (short)f32_to_bf16(d);
+  return r;
+}
+
+/* Features of one conserved state (FP32), before standardisation. */
+__device__ __forceinline__ void mlp_features(const double *U, double gamma, float *x)
+{
+  const double D = U[0], tau = U[1];
+  const double S2 = U[2]*U[2] + U[3]*U[3] + U[4]*U[4];
+  const double B2 = U[5]*U[5] + U[6]*U[6] + U[7]*U[7];
+  const double SB = U[2]*U[5] + U[3]*U[6] + U[4]*U[7];
+  const double iD = 1.0 / D;
+  x[0] = (float)log(fmax(tau * iD, 1e-30));
+  x[1] = (float)log1p(S2 * iD * iD);
+  x[2] = (float)log1p(B2 * iD);
+  x[3] = (float)(SB / (sqrt(S2 * B2) + 1e-300));
+  const double E = tau + D, S = sqrt(S2);
+  x[4] = (float)(2.0 * log(E) - log(fmax((E - S) * (E + S), 1e-300)));
+  x[5] = (float)gamma;
+}
+
+/* One wave evaluates the network on 64 samples. xin: standardised features of the
+ * wave's samples in LDS, [64][MLP_KIN] bf16 bits (columns >= MLP_IN zero); wf: forward
+ * fragments (LDS or global); y: per lane, the two raw outputs of the sample whose index
+ * equals the lane (returned through LDS scratch yout[64][2]).
+ * If act != nullptr the pre-activations are stored for back-propagation:
+ * act[layer][feature][sample] (f32, 3 x 64 x 64), samples local to the wave. */
+__device__ __forceinline__ void mlp_forward_wave(const uint16_t *xin, const uint16_t *wf,
+                                                 const float *b1, const float *b2, const float *b3,
+                                                 const float *b4, float *yout, float *act)
+{
+  const int lane = threadIdx.x & 63;
+  const int grp = lane >> 4, col = lane & 15;
+  bf16x4_t H[4][4];       // [k-tile][n-tile] B operands of the next layer
+  // layer 1 input fragments: lane holds features 4*grp..4*grp+3 of sample 16*nt + col
+  bf16x4_t X[4];
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(xin + (16 * nt + col) * MLP_KIN + 4 * grp);
+    X[nt][0] = (short)(v.x & 0xffff);
+    X[nt][1] = (short)(v.x >> 16);
+    X[nt][2] = (short)(v.y & 0xffff);
+    X[nt][3] = (short)(v.y >> 16);
+  }
+  // layer 1: 64 x 16 . 16 x 64
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    const bf16x4_t a = load_frag(wf, MLP_F1 + mt, lane);
+    float bias[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) bias[r] = b1[16 * mt + 4 * grp + r];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+      f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+      c = mfma16(a, X[nt], c);
+      float h[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float z = c[r] + bias[r];
+        if (act) act[(0 * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = z;
+        h[r] = silu(z);
+      }
+      H[mt][nt] = pack4(h[0], h[1], h[2], h[3]);
+    }
+  }
+  // layers 2 and 3: 64 x 64 . 64 x 64
+#pragma unroll
+  for (int layer = 0; layer < 2; ++layer) {
+    const int f0 = layer == 0 ? MLP_F2 : MLP_F3;
+    const float *bb = layer == 0 ? b2 : b3;
+    f32x4_t C[4][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) C[mt][nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt) {
+        const bf16x4_t a = load_frag(wf, f0 + 4 * mt + kt, lane);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) C[mt][nt] = mfma16(a, H[kt][nt], C[mt][nt]);
+      }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      float bias[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) bias[r] = bb[16 * mt + 4 * grp + r];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        float h[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float z = C[mt][nt][r] + bias[r];
+          if (act) act[((layer + 1) * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = z;
+          h[r] = silu(z);
+        }
+        H[mt][nt] = pack4(h[0], h[1], h[2], h[3]);
+      }
+    }
+  }
+  // layer 4: 16 (2 used) x 64 . 64 x 64
+#pragma unroll
+  for (int nt = 0; nt < 4; ++nt) {
+    f32x4_t c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) c = mfma16(load_frag(wf, MLP_F4 + kt, lane), H[kt][nt], c);
+    if (grp == 0) {
+      yout[(16 * nt + col) * 2 + 0] = c[0] + b4[0];
+      yout[(16 * nt + col) * 2 + 1] = c[1] + b4[1];
+    }
+  }
+}
+
+}  // namespace maragpu

@@ -37,6 +37,12 @@ EXPORTS = [
     "mara_config_solver_num", "mara_config_solver_bool", "mara_set_domain", "mara_advance",
     "mara_get_timestep", "mara_set_primitive", "mara_measure_mean_max_divB", "mara_last_error",
     "mara_device_context", "mara_init_turbulence",
+    "maragpu_mlp_create", "maragpu_mlp_destroy", "maragpu_mlp_param_count",
+    "maragpu_mlp_serialized_size", "maragpu_mlp_save", "maragpu_mlp_load", "maragpu_mlp_get_params",
+    "maragpu_mlp_set_params", "maragpu_mlp_forward", "maragpu_mlp_grad", "maragpu_mlp_train",
+    "maragpu_attach_mlp", "maragpu_batch_cons_to_prim_learned", "maragpu_mlp_data_create",
+    "maragpu_mlp_data_destroy", "maragpu_mlp_data_size", "maragpu_mlp_data_add_synthetic",
+    "maragpu_mlp_data_add_state",
 ]
 
 
@@ -66,6 +72,24 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class MlpSampling(C.Structure):
+    _fields_ = [("W_max", C.c_double), ("sigma_min", C.c_double), ("sigma_max", C.c_double),
+                ("ptr_min", C.c_double), ("ptr_max", C.c_double)]
+
+
+class MlpTrainConfig(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double),
+                ("lr_final", C.c_double)]
+
+
+class MlpTrainReport(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double), ("steps", C.c_int64),
+                ("samples", C.c_int64), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ICParams(C.Structure):
     _fields_ = [("N", C.c_int * 3), ("Ng", C.c_int), ("rho", C.c_double), ("pre", C.c_double),
                 ("B0", C.c_double * 3), ("v_rms", C.c_double), ("v_kmin", C.c_double),
@@ -74,6 +98,7 @@ class ICParams(C.Structure):
 
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int)
 _vp = C.c_void_p
 _sz = C.c_size_t
@@ -125,6 +150,27 @@ def lib():
         "mara_last_error": (C.c_char_p, []),
         "mara_device_context": (_vp, []),
         "mara_init_turbulence": (C.c_int, [C.POINTER(ICParams), _dp, _sz]),
+        "maragpu_mlp_create": (C.c_int, [C.POINTER(_vp), C.c_uint64]),
+        "maragpu_mlp_destroy": (C.c_int, [_vp]),
+        "maragpu_mlp_param_count": (_sz, []),
+        "maragpu_mlp_serialized_size": (_sz, []),
+        "maragpu_mlp_save": (C.c_int, [_vp, C.c_void_p, _sz]),
+        "maragpu_mlp_load": (C.c_int, [C.POINTER(_vp), C.c_void_p, _sz]),
+        "maragpu_mlp_get_params": (C.c_int, [_vp, _fp, _fp]),
+        "maragpu_mlp_set_params": (C.c_int, [_vp, _fp, _fp]),
+        "maragpu_mlp_forward": (C.c_int, [_vp, _sz, _fp, _fp]),
+        "maragpu_mlp_grad": (C.c_int, [_vp, _sz, _fp, _fp, _fp, _fp]),
+        "maragpu_mlp_train": (C.c_int, [_vp, _vp, C.POINTER(MlpTrainConfig),
+                                         C.POINTER(MlpTrainReport)]),
+        "maragpu_mlp_data_create": (C.c_int, [C.POINTER(_vp), C.c_int64]),
+        "maragpu_mlp_data_destroy": (C.c_int, [_vp]),
+        "maragpu_mlp_data_size": (C.c_int64, [_vp]),
+        "maragpu_mlp_data_add_synthetic": (C.c_int, [_vp, C.c_int64, C.c_uint64, C.c_double,
+                                                     C.POINTER(MlpSampling)]),
+        "maragpu_mlp_data_add_state": (C.c_int, [_vp, _vp, C.c_int64, C.c_uint64, C.c_double]),
+        "maragpu_attach_mlp": (C.c_int, [_vp, _vp]),
+        "maragpu_batch_cons_to_prim_learned": (C.c_int, [_vp, _sz, _dp, _dp, C.c_double, C.c_double,
+                                                         _ip, _ip, _ip, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -142,6 +188,11 @@ def dptr(a: np.ndarray):
 def iptr(a: np.ndarray):
     assert a.dtype == np.int32 and a.flags.c_contiguous
     return a.ctypes.data_as(_ip)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_fp)
 
 
 def check(rc: int, what: str = "maragpu"):
@@ -218,6 +269,11 @@ class Context:
         m, x, a = C.c_double(), C.c_double(), C.c_double()
         check(lib().maragpu_measure_divB(self.h, C.byref(m), C.byref(x), C.byref(a)), "divB")
         return m.value, x.value, a.value
+
+    def attach_mlp(self, net: "Mlp"):
+        """Use `net` for the learned c2p guess (the context keeps a reference)."""
+        self._mlp = net
+        check(lib().maragpu_attach_mlp(self.h, net.h if net is not None else None), "attach_mlp")
 
     def kernel_times(self):
         f, c, u = C.c_double(), C.c_double(), C.c_double()
@@ -296,3 +352,138 @@ def init_turbulence(N, Ng=3, rho=1.0, pre=1.0, B0=(0.0, 0.0, 1.0), v_rms=0.3, v_
     if rc != 0:
         raise MaraGpuError(f"mara_init_turbulence failed ({rc})")
     return P
+
+
+# ---------------------------------------------------------------------------
+# learned c2p network
+# ---------------------------------------------------------------------------
+class Mlp:
+    """The learned c2p network (maragpu_mlp): device-resident parameters."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def create(cls, seed=1):
+        h = _vp()
+        check(lib().maragpu_mlp_create(C.byref(h), seed), "mlp_create")
+        return cls(h)
+
+    @classmethod
+    def from_bytes(cls, data: bytes):
+        h = _vp()
+        buf = C.create_string_buffer(data, len(data))
+        check(lib().maragpu_mlp_load(C.byref(h), buf, len(data)), "mlp_load")
+        return cls(h)
+
+    @classmethod
+    def load(cls, path):
+        with open(path, "rb") as f:
+            return cls.from_bytes(f.read())
+
+    def to_bytes(self) -> bytes:
+        n = lib().maragpu_mlp_serialized_size()
+        buf = C.create_string_buffer(n)
+        check(lib().maragpu_mlp_save(self.h, buf, n), "mlp_save")
+        return buf.raw
+
+    def save(self, path):
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        with open(path, "wb") as f:
+            f.write(self.to_bytes())
+
+    def close(self):
+        if self.h:
+            lib().maragpu_mlp_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        th = np.zeros(lib().maragpu_mlp_param_count(), dtype=np.float32)
+        nm = np.zeros(16, dtype=np.float32)
+        check(lib().maragpu_mlp_get_params(self.h, fptr(th), fptr(nm)), "mlp_get_params")
+        return th, nm
+
+    def set_params(self, theta=None, norms=None):
+        th = None if theta is None else np.ascontiguousarray(theta, dtype=np.float32)
+        nm = None if norms is None else np.ascontiguousarray(norms, dtype=np.float32)
+        check(lib().maragpu_mlp_set_params(self.h, fptr(th) if th is not None else None,
+                                           fptr(nm) if nm is not None else None), "mlp_set_params")
+
+    def forward(self, Xs):
+        Xs = np.ascontiguousarray(Xs, dtype=np.float32).reshape(-1, 6)
+        Y = np.zeros((len(Xs), 2), dtype=np.float32)
+        check(lib().maragpu_mlp_forward(self.h, len(Xs), fptr(Xs), fptr(Y)), "mlp_forward")
+        return Y
+
+    def grad(self, Xs, Ts):
+        Xs = np.ascontiguousarray(Xs, dtype=np.float32).reshape(-1, 6)
+        Ts = np.ascontiguousarray(Ts, dtype=np.float32).reshape(-1, 2)
+        g = np.zeros(lib().maragpu_mlp_param_count(), dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        check(lib().maragpu_mlp_grad(self.h, len(Xs), fptr(Xs), fptr(Ts), fptr(g), fptr(loss)),
+              "mlp_grad")
+        return g, float(loss[0])
+
+    def train(self, data: "MlpData", epochs=2, batch=8192, lr=2e-3, lr_final=1e-5, seed=7):
+        cfg = MlpTrainConfig(seed, epochs, batch, lr, lr_final)
+        rep = MlpTrainReport()
+        check(lib().maragpu_mlp_train(self.h, data.h, C.byref(cfg), C.byref(rep)), "mlp_train")
+        return rep.as_dict()
+
+    def cons_to_prim(self, U, gamma, pfloor=-1.0, guess=None):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 8)
+        P = np.zeros_like(U) if guess is None else np.array(guess, dtype=np.float64).reshape(-1, 8)
+        err = np.zeros(len(U), dtype=np.int32)
+        it = np.zeros(len(U), dtype=np.int32)
+        used = np.zeros(len(U), dtype=np.int32)
+        start = np.zeros((len(U), 2))
+        check(lib().maragpu_batch_cons_to_prim_learned(self.h, len(U), dptr(U), dptr(P), gamma,
+                                                       pfloor, iptr(err), iptr(it), iptr(used),
+                                                       dptr(start)),
+              "cons_to_prim_learned")
+        return P, err, it, used, start
+
+
+class MlpData:
+    """Training pairs held on the device (maragpu_mlp_data)."""
+
+    def __init__(self, capacity):
+        self.h = _vp()
+        check(lib().maragpu_mlp_data_create(C.byref(self.h), int(capacity)), "mlp_data_create")
+
+    def close(self):
+        if self.h:
+            lib().maragpu_mlp_data_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return int(lib().maragpu_mlp_data_size(self.h))
+
+    def add_synthetic(self, n, seed=1, gamma=4.0 / 3.0, W_max=50.0, sigma=(0.01, 100.0),
+                      ptr=(1e-4, 10.0)):
+        r = MlpSampling(W_max, sigma[0], sigma[1], ptr[0], ptr[1])
+        check(lib().maragpu_mlp_data_add_synthetic(self.h, int(n), seed, gamma, C.byref(r)),
+              "mlp_data_add_synthetic")
+
+    def add_state(self, ctx: Context, n, seed=1, gamma=4.0 / 3.0):
+        check(lib().maragpu_mlp_data_add_state(self.h, ctx.h, int(n), seed, gamma),
+              "mlp_data_add_state")
+
+
+def mlp_load(ctx: Context, path: str) -> Mlp:
+    """Loads a saved network and attaches it to ctx for the learned c2p guess."""
+    net = Mlp.load(path)
+    ctx.attach_mlp(net)
+    return net

@@ -116,6 +116,79 @@ int maragpu_batch_reconstruct(size_t n, const double *V, double *out, int op, do
 int maragpu_batch_riemann(size_t n, int solver, int precision, const double *PL, const double *PR,
                           double *F, int dim, double gamma, double pfloor, int *err, double *lam);
 
+/* ---------------------------------------------------------------------------------
+ * Learned c2p initial guess (no counterpart in the reference: the reference starts its
+ * Newton solves from the cons-based estimate, Mara/rmhd-c2p.c:97-101, and from the
+ * previous primitive state). A 6 -> 64 -> 64 -> 64 -> 2 SiLU network, BF16 weights and
+ * FP32 accumulation on the matrix cores, maps scale-free features of U to log W and
+ * log(p/D); the FP64 anton2dzw Newton solve polishes every zone from that guess and the
+ * plain cascade is the fallback.
+ * ------------------------------------------------------------------------------- */
+typedef struct maragpu_mlp maragpu_mlp;
+
+/* Training-pair sampling ranges (log-uniform): Lorentz factor 1 .. W_max,
+ * magnetisation sigma = b^2/(rho h), p/rho; rho log-uniform on [0.1, 10], isotropic
+ * velocity and field directions (random field-velocity angle). */
+typedef struct maragpu_mlp_sampling {
+  double W_max;
+  double sigma_min, sigma_max;
+  double ptr_min, ptr_max;
+} maragpu_mlp_sampling;
+
+typedef struct maragpu_mlp_train_config {
+  uint64_t seed;           /* minibatch draws */
+  int epochs;              /* passes of (pairs / batch) steps */
+  int batch;               /* multiple of 256 */
+  double lr, lr_final;     /* Adam, cosine decay */
+} maragpu_mlp_train_config;
+
+typedef struct maragpu_mlp_train_report {
+  double loss_first, loss_last;  /* mean squared error of standardised targets */
+  int64_t steps, samples;
+  double seconds;
+} maragpu_mlp_train_report;
+
+int maragpu_mlp_create(maragpu_mlp **out, uint64_t seed);
+int maragpu_mlp_destroy(maragpu_mlp *net);
+size_t maragpu_mlp_param_count(void);
+size_t maragpu_mlp_serialized_size(void);
+int maragpu_mlp_save(const maragpu_mlp *net, void *buf, size_t nbytes);
+int maragpu_mlp_load(maragpu_mlp **out, const void *buf, size_t nbytes);
+/* theta: maragpu_mlp_param_count() floats; norms: 16 floats (x mean[6], x std[6],
+ * y mean[2], y std[2]) */
+int maragpu_mlp_get_params(const maragpu_mlp *net, float *theta, float *norms);
+int maragpu_mlp_set_params(maragpu_mlp *net, const float *theta, const float *norms);
+/* raw network outputs for standardised inputs (n x 6 -> n x 2) */
+int maragpu_mlp_forward(const maragpu_mlp *net, size_t n, const float *Xs, float *Y);
+/* gradient of the mean squared error over n (multiple of 256) standardised samples */
+int maragpu_mlp_grad(maragpu_mlp *net, size_t n, const float *Xs, const float *Ts, float *grad,
+                     float *loss);
+/* Training pairs (conserved -> primitive) held on the device. */
+typedef struct maragpu_mlp_data maragpu_mlp_data;
+int maragpu_mlp_data_create(maragpu_mlp_data **out, int64_t capacity);
+int maragpu_mlp_data_destroy(maragpu_mlp_data *data);
+int64_t maragpu_mlp_data_size(const maragpu_mlp_data *data);
+/* appends up to n pairs sampled over the ranges (states that fail prim_to_cons are dropped) */
+int maragpu_mlp_data_add_synthetic(maragpu_mlp_data *data, int64_t n, uint64_t seed, double gamma,
+                                   const maragpu_mlp_sampling *ranges);
+/* appends up to n pairs drawn (with replacement) from the interior zones of ctx's
+ * committed primitive state: a snapshot of the run */
+int maragpu_mlp_data_add_state(maragpu_mlp_data *data, maragpu_ctx *ctx, int64_t n, uint64_t seed,
+                               double gamma);
+/* sets the standardisation from the data and trains (Adam) on the device */
+int maragpu_mlp_train(maragpu_mlp *net, const maragpu_mlp_data *data, const maragpu_mlp_train_config *cfg,
+                      maragpu_mlp_train_report *report);
+/* the network a context uses for MARAGPU_C2P_LEARNED (not owned; NULL detaches) */
+int maragpu_attach_mlp(maragpu_ctx *ctx, const maragpu_mlp *net);
+/* learned-guess c2p on n states (host arrays): P holds the previous primitive states
+ * (the fallback cascade's second guess) on entry and receives the primitives; iters the
+ * Newton iterations, err the error code, used_guess 1 where the solve from the network's
+ * guess converged (0 where the plain cascade took over); start (2n doubles, may be NULL)
+ * the Newton start (Z, W) built from the network's outputs */
+int maragpu_batch_cons_to_prim_learned(const maragpu_mlp *net, size_t n, const double *U, double *P,
+                                       double gamma, double pfloor, int *err, int *iters,
+                                       int *used_guess, double *start);
+
 #ifdef __cplusplus
 }
 #endif

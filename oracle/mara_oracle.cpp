@@ -492,6 +492,24 @@ void oracle_c2p_solver_batch(long n, int which, const double *U, double *P, doub
   }
 }
 
+/* anton2dzw from a given Newton start (Z, W) per state (start: 2n doubles); the
+ * reference reaches the same solve through rmhd_c2p_set_starting_prim (Mara/rmhd-c2p.c:108-119),
+ * which sets Z_start and W_start from a primitive state. Checker for the learned guess. */
+void oracle_c2p_from_start_batch(long n, const double *U, const double *start, double *P, double gamma,
+                                 double pfloor, int *err, int *iters)
+{
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long z = 0; z < n; ++z) {
+    C2P<double> s(gamma, pfloor);
+    s.new_state(U + 8 * z);
+    s.Z_start = start[2 * z + 0];
+    s.W_start = start[2 * z + 1];
+    int e = s.solve_anton2dzw(P + 8 * z);
+    if (err) err[z] = e;
+    if (iters) iters[z] = s.iterations;
+  }
+}
+
 void oracle_flux_eigen_batch(long n, const double *P, double *F, double *ap, double *am,
                              int dim, double gamma)
 {

@@ -1,0 +1,133 @@
+"""ORACLE — test infrastructure only.
+
+CPU evaluation (numpy) of the learned-c2p network and of the gradient of its training
+loss, with the device's precision choices restated: weights, layer inputs and the
+back-propagated pre-activation gradients are rounded to bfloat16 (round to nearest
+even) wherever the device feeds them to a matrix-core product; sums are accumulated in
+float64 here (float32 on the device). The standardised input x enters the first layer
+as hi = bf16(x) plus lo = bf16(x - hi) through one shared weight column, so the first
+product sees hi + lo. Parameter order matches ctf_amd/csrc/mlp.hip: W1[64][6], b1[64],
+W2[64][64], b2[64], W3[64][64], b3[64], W4[2][64], b4[2].
+
+The network and its loss have no counterpart in the reference (Mara); this module is the
+checker for the device's forward and backward kernels.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SHAPES = [("W1", (64, 6)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 64)),
+          ("b3", (64,)), ("W4", (2, 64)), ("b4", (2,))]
+NPARAM = sum(int(np.prod(s)) for _, s in SHAPES)
+
+
+def bf16(x):
+    """Round float32 values to bfloat16 (nearest even), returned as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    r = np.where(nan, (u | 0x400000), r).astype(np.uint32)
+    return r.view(np.float32).reshape(a.shape)
+
+
+def unpack(theta):
+    theta = np.asarray(theta)
+    if theta.dtype != np.float64:
+        theta = theta.astype(np.float32)
+    out, off = {}, 0
+    for name, shape in SHAPES:
+        n = int(np.prod(shape))
+        out[name] = theta[off:off + n].reshape(shape)
+        off += n
+    return out
+
+
+def pack(d):
+    return np.concatenate([np.asarray(d[n], dtype=np.float64).ravel() for n, _ in SHAPES])
+
+
+def silu(a):
+    return a / (1.0 + np.exp(-a))
+
+
+def silu_grad(a):
+    s = 1.0 / (1.0 + np.exp(-a))
+    return s * (1.0 + a * (1.0 - s))
+
+
+def split_input(Xs):
+    """hi + lo of the first layer's input: hi = bf16(x), lo = bf16(x - hi), x in float32."""
+    x = np.asarray(Xs, dtype=np.float32)
+    hi = bf16(x)
+    lo = bf16((x - hi).astype(np.float32))
+    return hi.astype(np.float64) + lo.astype(np.float64)
+
+
+def _q(exact):
+    """bfloat16 rounding as the device does it, or the identity (exact FP64 network)."""
+    if exact:
+        return lambda v: np.asarray(v, dtype=np.float64)
+    return lambda v: bf16(np.asarray(v, dtype=np.float32)).astype(np.float64)
+
+
+def forward(theta, Xs, keep=False, exact=False):
+    q = _q(exact)
+    p = unpack(theta) if not exact else {k: v.astype(np.float64) for k, v in unpack(theta).items()}
+    W = {k: q(p[k]) for k in ("W1", "W2", "W3", "W4")}
+    X = np.asarray(Xs, dtype=np.float64) if exact else split_input(Xs)
+    A1 = X @ W["W1"].T + p["b1"]
+    H1 = q(silu(A1))
+    A2 = H1 @ W["W2"].T + p["b2"]
+    H2 = q(silu(A2))
+    A3 = H2 @ W["W3"].T + p["b3"]
+    H3 = q(silu(A3))
+    Y = H3 @ W["W4"].T + p["b4"]
+    if keep:
+        return Y, dict(X=X, A1=A1, H1=H1, A2=A2, H2=H2, A3=A3, H3=H3, W=W)
+    return Y
+
+
+def loss_and_grad(theta, Xs, Ts, exact=False):
+    """Mean squared error over n samples x 2 outputs and its gradient (NPARAM values).
+    exact=True drops every bfloat16 rounding (the FP64 network)."""
+    q = _q(exact)
+    Y, c = forward(theta, Xs, keep=True, exact=exact)
+    n = len(Xs)
+    E = Y - np.asarray(Ts, dtype=np.float64)
+    loss = float(np.mean(E * E))
+    dY = q(2.0 * E / (2.0 * n))
+    W = c["W"]
+    g = {}
+    g["W4"] = dY.T @ c["H3"]
+    g["b4"] = dY.sum(0)
+    dZ3 = q(dY @ W["W4"] * silu_grad(c["A3"]))
+    g["W3"] = dZ3.T @ c["H2"]
+    g["b3"] = dZ3.sum(0)
+    dZ2 = q(dZ3 @ W["W3"] * silu_grad(c["A2"]))
+    g["W2"] = dZ2.T @ c["H1"]
+    g["b2"] = dZ2.sum(0)
+    dZ1 = q(dZ2 @ W["W2"] * silu_grad(c["A1"]))
+    g["W1"] = dZ1.T @ c["X"]
+    g["b1"] = dZ1.sum(0)
+    return loss, pack(g)
+
+
+def features(U, gamma):
+    """Scale-free features of conserved states (n x 8) -> n x 6 (before standardisation)."""
+    U = np.asarray(U, dtype=np.float64)
+    D, tau = U[:, 0], U[:, 1]
+    S2 = (U[:, 2:5] ** 2).sum(1)
+    B2 = (U[:, 5:8] ** 2).sum(1)
+    SB = (U[:, 2:5] * U[:, 5:8]).sum(1)
+    iD = 1.0 / D
+    x = np.zeros((len(U), 6))
+    x[:, 0] = np.log(np.maximum(tau * iD, 1e-30))
+    x[:, 1] = np.log1p(S2 * iD * iD)
+    x[:, 2] = np.log1p(B2 * iD)
+    x[:, 3] = SB / (np.sqrt(S2 * B2) + 1e-300)
+    E = tau + D
+    S = np.sqrt(S2)
+    x[:, 4] = 2.0 * np.log(E) - np.log(np.maximum((E - S) * (E + S), 1e-300))
+    x[:, 5] = gamma
+    return x

@@ -136,6 +136,8 @@ class Lib:
             L.oracle_get_threads.restype = C.c_int
             L.oracle_mean_max_divB.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
             L.oracle_count_flops.argtypes = [C.c_void_p, _dp, C.c_long, _dp]
+            L.oracle_c2p_from_start_batch.argtypes = [C.c_long, _dp, _dp, _dp, C.c_double,
+                                                      C.c_double, _ip, _ip]
 
     def _f(self, name):
         return getattr(self.lib, self.p + name)
@@ -163,6 +165,17 @@ class Lib:
         err = np.zeros(len(U), dtype=np.int32)
         it = np.zeros(len(U), dtype=np.int32)
         self._f("c2p_solver_batch")(len(U), which, _d(U), _d(P), gamma, pfloor, _i(err), _i(it))
+        return P, err, it
+
+    def c2p_from_start(self, U, start, gamma, pfloor=-1.0):
+        """anton2dzw from the Newton start (Z, W) of each state (oracle only)."""
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(-1, 8)
+        start = np.ascontiguousarray(start, dtype=np.float64).reshape(-1, 2)
+        P = np.zeros_like(U)
+        err = np.zeros(len(U), dtype=np.int32)
+        it = np.zeros(len(U), dtype=np.int32)
+        self.lib.oracle_c2p_from_start_batch(len(U), _d(U), _d(start), _d(P), gamma, pfloor,
+                                             _i(err), _i(it))
         return P, err, it
 
     def flux_eigen(self, P, dim, gamma):

@@ -1,0 +1,147 @@
+"""The learned c2p initial guess on the device: the BF16 matrix-core network against its
+CPU oracle (forward and training gradients), training on the GPU, and the guess inside
+the conserved-to-primitive recovery (same answers as the plain cascade, fewer Newton
+iterations).
+
+Tolerances: the device accumulates in FP32 (the oracle in FP64) and uses a fast exp in
+SiLU, so a layer input occasionally rounds to the neighbouring bfloat16 value; outputs
+agree to 2e-3 (relative to 1 + |y|) and gradients to 1e-2 in relative L2 norm per
+parameter block. Recovered primitives must match the plain cascade to 1e-9 (both stop
+on a 1e-12 Newton step).
+"""
+import sys
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 4.0 / 3.0
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+
+
+@pytest.fixture(scope="module")
+def trained(gpu):
+    data = gpu.MlpData(1_000_000)
+    data.add_synthetic(1_000_000, seed=11, W_max=20.0, sigma=(0.01, 10.0), ptr=(1e-3, 10.0))
+    net = gpu.Mlp.create(seed=3)
+    rep = net.train(data, epochs=6, batch=1024, lr=3e-3, lr_final=1e-5, seed=12)
+    return net, rep
+
+
+def blocks(v):
+    out, off = {}, 0
+    for name, shape in mo.SHAPES:
+        n = int(np.prod(shape))
+        out[name] = v[off:off + n]
+        off += n
+    return out
+
+
+def test_forward_matches_oracle(gpu):
+    net = gpu.Mlp.create(seed=5)
+    rng = np.random.default_rng(0)
+    th = (rng.normal(size=mo.NPARAM) * 0.25).astype(np.float32)
+    net.set_params(th)
+    X = rng.normal(size=(1000, 6)).astype(np.float32)
+    Yg = net.forward(X)
+    Yo = mo.forward(th, X)
+    d = np.abs(Yg - Yo) / (1.0 + np.abs(Yo))
+    assert np.mean(d < 2e-3) > 0.999 and d.max() < 2e-2, d.max()
+
+
+def test_gradient_matches_oracle(gpu):
+    net = gpu.Mlp.create(seed=6)
+    rng = np.random.default_rng(1)
+    th = (rng.normal(size=mo.NPARAM) * 0.2).astype(np.float32)
+    net.set_params(th)
+    X = rng.normal(size=(1024, 6)).astype(np.float32)
+    T = rng.normal(size=(1024, 2)).astype(np.float32)
+    g, loss = net.grad(X, T)
+    lo, go = mo.loss_and_grad(th, X, T)
+    assert abs(loss - lo) <= 1e-3 * lo
+    bg, bo = blocks(g), blocks(go)
+    for name in bo:
+        rel = np.linalg.norm(bg[name] - bo[name]) / np.linalg.norm(bo[name])
+        assert rel < 1e-2, (name, rel)
+
+
+def test_training_reduces_loss(trained):
+    net, rep = trained
+    assert rep["steps"] > 1000 and rep["samples"] > 990_000
+    assert rep["loss_last"] < 0.01 * rep["loss_first"], rep
+
+
+def test_learned_guess_recovers_the_same_primitives(gpu, oracle_lib, trained):
+    net, _ = trained
+    from make_golden import sample_states
+    P = sample_states(np.random.default_rng(4), 4000, Wmax=20.0, sig=(0.01, 10.0), ptr=(1e-3, 10.0))
+    U, _ = oracle_lib.prim_to_cons(P, GAMMA)
+    Po, eo, io = oracle_lib.cons_to_prim(U, GAMMA)          # plain cascade, zero previous state
+    Pl, el, il, used, start = net.cons_to_prim(U, GAMMA)
+    assert np.array_equal(el == 0, eo == 0)
+    ok = eo == 0
+    rel = np.abs(Pl[ok] - Po[ok]).max(1) / np.abs(Po[ok]).max(1)
+    assert rel.max() < 1e-9
+    assert used[ok].mean() > 0.99
+    # the polish itself: the oracle's anton2dzw from the device's Newton start agrees
+    Ps, es, its = oracle_lib.c2p_from_start(U, start, GAMMA)
+    g = ok & (used == 1)
+    assert np.all(es[g] == 0)
+    assert np.abs(Ps[g] - Pl[g]).max() / np.abs(Ps[g]).max() < 1e-10
+    assert np.mean(np.abs(its[g] - il[g]) <= 1) > 0.99
+    # what the network buys: fewer Newton iterations per recovery
+    assert il[ok].mean() < 0.7 * io[ok].mean(), (il[ok].mean(), io[ok].mean())
+
+
+def test_learned_c2p_in_full_steps(gpu, trained):
+    net, _ = trained
+    N = 24
+    P0 = gpu.init_turbulence((N, N, N), v_rms=0.5, v_kmax=6.0, four_velocity=True, dB_rms=0.4,
+                             A_kmax=4.0, seed=17)
+    out = {}
+    for c2p in ("plain", "learned"):
+        ctx = gpu.Context(gpu.make_config(N=(N, N, N), extrap="weno5", riemann="hlld", advance="rk3",
+                                          c2p=c2p))
+        if c2p == "learned":
+            ctx.attach_mlp(net)
+        ctx.upload(P0)
+        dt = 0.0
+        iters = zones = guesses = fallbacks = 0
+        for _ in range(3):
+            e, st = ctx.advance(dt)
+            assert e == 0
+            iters += st.c2p_iterations
+            zones += st.c2p_zones
+            guesses += st.mlp_guesses
+            fallbacks += st.mlp_fallbacks
+            dt = ctx.get_timestep(0.4)
+        out[c2p] = (ctx.download(), iters / zones, guesses, fallbacks)
+    it = (slice(3, -3),) * 3
+    a, b = out["learned"][0][it], out["plain"][0][it]
+    L1 = np.abs(a - b).sum(axis=(0, 1, 2)) / np.abs(b).sum(axis=(0, 1, 2))
+    assert L1.max() < 1e-10, L1
+    assert out["learned"][2] > 0 and out["learned"][3] < 0.05 * out["learned"][2]
+    assert out["learned"][1] < out["plain"][1]
+
+
+def test_learned_c2p_requires_a_network(gpu):
+    N = 8
+    ctx = gpu.Context(gpu.make_config(N=(N, N, N), c2p="learned"))
+    ctx.upload(gpu.init_turbulence((N, N, N), v_rms=0.2, seed=2))
+    with pytest.raises(gpu.MaraGpuError):
+        ctx.advance(0.0)
+
+
+def test_save_and_load_round_trip(gpu, trained):
+    net, _ = trained
+    blob = net.to_bytes()
+    net2 = gpu.Mlp.from_bytes(blob)
+    X = np.random.default_rng(3).normal(size=(512, 6)).astype(np.float32)
+    assert np.array_equal(net.forward(X), net2.forward(X))
+    th1, nm1 = net.params()
+    th2, nm2 = net2.params()
+    assert np.array_equal(th1, th2) and np.array_equal(nm1, nm2)
