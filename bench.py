@@ -55,8 +55,10 @@ def interfaces_per_sweep(w):
     return [T - 5 * s for s in strides]
 
 
-def run_form(w, P0, steps, warmup, dts=None, learned_model=None):
-    """Runs one form; returns timings, stats, the final state and the dt sequence."""
+def run_form(w, P0, steps, warmup, dts=None, learned_model=None, total_steps=0):
+    """Runs one form; returns timings, stats, the state after total_steps steps (the
+    priming step not counted; untimed steps follow the timed ones up to that count) and
+    the dt sequence."""
     cfg = native.make_config(**w.config_kwargs())
     ctx = native.Context(cfg)
     if learned_model is not None:
@@ -99,6 +101,12 @@ def run_form(w, P0, steps, warmup, dts=None, learned_model=None):
         for key, val in st.as_dict().items():
             agg[key] = agg.get(key, 0) + val
     wall = time.perf_counter() - t0
+    done = warmup + steps
+    while done < total_steps:
+        e, _ = ctx.advance(next_dt())
+        if e:
+            raise RuntimeError(f"{w.name}: step {done + 1} failed on {e} zones")
+        done += 1
     divB = ctx.divB()
     P = ctx.download()
     ctx.close()
@@ -118,6 +126,7 @@ def run_form(w, P0, steps, warmup, dts=None, learned_model=None):
         "max_divB": divB[2],
         "P": P,
         "dts": used,
+        "steps_run": done,
     }
 
 
@@ -174,8 +183,10 @@ def traffic_record(w):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--accuracy-steps", type=int, default=None,
+                    help="steps after which L1 and div B are measured (default: the workload's)")
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--grid", type=int, default=None, help="override the workload's grid size")
     ap.add_argument("--forms", default="plain,mixed,learned")
@@ -211,18 +222,21 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     results = {}
     log("[bench] plain FP64")
-    plain = run_form(w, P0, args.steps, args.warmup)
+    acc_steps = w.steps if args.accuracy_steps is None else args.accuracy_steps
+    plain = run_form(w, P0, args.steps, args.warmup, total_steps=acc_steps)
     results["plain"] = plain
     dts = plain["dts"]
     if "mixed" in forms:
         log("[bench] mixed precision")
         wm = workloads.replace(w, precision="mixed")
-        results["mixed"] = run_form(wm, P0, args.steps, args.warmup, dts=dts)
-    if "learned" in forms and os.path.exists(args.model) and hasattr(native, "mlp_load"):
+        results["mixed"] = run_form(wm, P0, args.steps, args.warmup, dts=dts, total_steps=acc_steps)
+    if "learned" in forms:
+        if not os.path.exists(args.model):
+            raise SystemExit(f"learned form requested but no network file at {args.model}")
         log("[bench] mixed + learned c2p")
         wl = workloads.replace(w, precision="mixed", c2p="learned")
         results["learned"] = run_form(wl, P0, args.steps, args.warmup, dts=dts,
-                                      learned_model=args.model)
+                                      learned_model=args.model, total_steps=acc_steps)
 
     fc = flop_counts(w)
     nif = interfaces_per_sweep(w)
@@ -248,6 +262,7 @@ def main():
                "flux_ms_per_step": r["flux_ms_per_step"], "c2p_ms_per_step": r["c2p_ms_per_step"],
                "update_ms_per_step": r["update_ms_per_step"], "max_divB": r["max_divB"],
                "priming_step_ms": r["prime_ms"]}
+        rec["accuracy_after_steps"] = r["steps_run"]
         if name != "plain":
             rec["L1_vs_fp64"] = l1_vs(r["P"], plain["P"], w.Ng)
         st = r["stats"]

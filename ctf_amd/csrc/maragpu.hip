@@ -198,16 +198,21 @@ __device__ __forceinline__ int scatter_images(const GridDesc &g, double *__restr
 
 /* Per-zone conserved->primitive (GodunovOperator::ConsToPrim, Mara/hydro.cpp:125-202)
  * with the emergency reset of :165-196. `first` = the previous primitive state is the
- * zero-filled PrimitiveArray (first RK stage). */
+ * zero-filled PrimitiveArray (first RK stage). LEARNED: each zone first runs anton2dzw
+ * from the network's Newton start (k_mlp_start); a zone whose solve fails runs the plain
+ * cascade. */
+template <bool LEARNED>
 __global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restrict__ U,
                                                          double *__restrict__ P, GridDesc g,
                                                          double gamma, double pfloor, int first,
                                                          int ereset, double preset, double dreset,
-                                                         DevStats *st, int pass)
+                                                         DevStats *st, int pass,
+                                                         const double2 *__restrict__ start)
 {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  constexpr int NC = LEARNED ? 6 : 4;
+  unsigned long long cnt[NC] = {};
   if (t < ni) {
     const long T = g.T;
     const int k = (int)(t % g.N[2]) + g.Ng;
@@ -220,8 +225,18 @@ __global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restric
       u[q] = U[q * T + z];
       p[q] = first ? 0.0 : P[q * T + z];
     }
-    int iters = 0, which = 0;
-    const int err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
+    int iters = 0, which = 0, err = 1, used = 0;
+    if (LEARNED) {
+      err = check_cons(u);
+      if (!err) {
+        C2PState s;
+        c2p_new_state(s, u, gamma, pfloor);
+        const double2 zw = start[t];
+        err = c2p_anton2dzw(s, zw.x, zw.y, p, iters);
+        used = err == 0;
+      }
+    }
+    if (err) err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
     int failed = err != 0;
     if (failed && ereset) {
       p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
@@ -231,12 +246,23 @@ __global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restric
     const int mult = scatter_images(g, P, T, i, j, k, p);
     cnt[0] = 1;
     cnt[1] = (unsigned long long)iters;
-    cnt[2] = which != 0;
+    cnt[2] = LEARNED ? !used : which != 0;
     cnt[3] = failed ? (unsigned long long)mult : 0ull;
+    if constexpr (LEARNED) {
+      cnt[NC - 2] = 1;
+      cnt[NC - 1] = !used;
+    }
   }
-  unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
-                                       &st->stage_failed[pass] };
-  block_add<4>(cnt, dst);
+  if constexpr (LEARNED) {
+    unsigned long long *const dst[6] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                         &st->stage_failed[pass], &st->mlp_guesses, &st->mlp_fallbacks };
+    block_add<6>(cnt, dst);
+  }
+  else {
+    unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                         &st->stage_failed[pass] };
+    block_add<4>(cnt, dst);
+  }
 }
 
 /* Newton start (Z, W) of the anton2dzw solve from the network's raw outputs: W = exp(y0)
@@ -254,16 +280,11 @@ __device__ __forceinline__ void learned_start(double D, float y0, float y1, cons
   Wg = W;
 }
 
-/* Per-zone conserved->primitive with the learned initial guess: the network (one wave
- * per 64 zones on the matrix cores) predicts log W and log(p/D); anton2dzw polishes
- * from (Z, W) of that guess in FP64; a zone whose polish fails runs the plain cascade
- * of k_cons_to_prim. Blocks loop over 256-zone chunks so the weights are staged into
- * LDS once per block. */
-__global__ __launch_bounds__(kBlock) void k_cons_to_prim_learned(const double *__restrict__ U,
-                                                                 double *__restrict__ P, GridDesc g,
-                                                                 double gamma, double pfloor, int first,
-                                                                 int ereset, double preset, double dreset,
-                                                                 DevStats *st, int pass, MlpDev net)
+/* The learned c2p's network pass over the interior: one wave per 64 zones on the matrix
+ * cores, writing each zone's Newton start (Z, W) (interior index order). Blocks loop over
+ * 256-zone chunks so the weights are staged into LDS once per block. */
+__global__ __launch_bounds__(kBlock) void k_mlp_start(const double *__restrict__ U, GridDesc g, double gamma,
+                                                      MlpDev net, double2 *__restrict__ start)
 {
   __shared__ uint16_t wf[MLP_NFWD * MLP_FRAG];
   __shared__ float bias[3 * 64 + 2];
@@ -276,26 +297,21 @@ __global__ __launch_bounds__(kBlock) void k_cons_to_prim_learned(const double *_
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long ni = (long)g.N[0] * g.N[1] * g.N[2];
   const long T = g.T;
-  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
   for (long base = (long)blockIdx.x * kBlock; base < ni; base += (long)gridDim.x * kBlock) {
     const long t = base + threadIdx.x;
     const bool valid = t < ni;
-    double u[8], p[8];
-    int i = 0, j = 0, k = 0;
+    double u[8];
     if (valid) {
-      k = (int)(t % g.N[2]) + g.Ng;
-      j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
-      i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
-      const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
+      const long k = t % g.N[2] + g.Ng;
+      const long j = (t / g.N[2]) % g.N[1] + g.Ng;
+      const long i = t / ((long)g.N[2] * g.N[1]) + g.Ng;
+      const long z = (i * g.n[1] + j) * g.n[2] + k;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        u[q] = U[q * T + z];
-        p[q] = first ? 0.0 : P[q * T + z];
-      }
+      for (int q = 0; q < 8; ++q) u[q] = U[q * T + z];
     }
     else {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) { u[q] = q == 0 ? 1.0 : (q == 1 ? 1.0 : 0.0); p[q] = 0.0; }
+      for (int q = 0; q < 8; ++q) u[q] = q < 2 ? 1.0 : 0.0;
     }
     float x[MLP_IN];
     mlp_features(u, gamma, x);
@@ -306,37 +322,12 @@ __global__ __launch_bounds__(kBlock) void k_cons_to_prim_learned(const double *_
     mlp_forward_wave(xin[wave], wf, bias, bias + 64, bias + 128, bias + 192, yout[wave], nullptr);
     __syncthreads();
     const float y0 = yout[wave][lane * 2 + 0], y1 = yout[wave][lane * 2 + 1];
-    __syncthreads();
     if (valid) {
-      int iters = 0, which = 0, used = 0;
-      int err = check_cons(u);
-      if (!err) {
-        C2PState s;
-        c2p_new_state(s, u, gamma, pfloor);
-        double Zg, Wg;
-        learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
-        err = c2p_anton2dzw(s, Zg, Wg, p, iters);
-        used = err == 0;
-        if (err) err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
-      }
-      int failed = err != 0;
-      if (failed && ereset) {
-        p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
-        p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
-        failed = 0;
-      }
-      const int mult = scatter_images(g, P, T, i, j, k, p);
-      cnt[0] += 1;
-      cnt[1] += (unsigned long long)iters;
-      cnt[2] += used ? 0ull : 1ull;
-      cnt[3] += failed ? (unsigned long long)mult : 0ull;
-      cnt[4] += 1;
-      cnt[5] += used ? 0ull : 1ull;
+      double Zg, Wg;
+      learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
+      start[t] = make_double2(Zg, Wg);
     }
   }
-  unsigned long long *const dst[6] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
-                                       &st->stage_failed[pass], &st->mlp_guesses, &st->mlp_fallbacks };
-  block_add<6>(cnt, dst);
 }
 
 __global__ __launch_bounds__(kBlock) void kb_cons_to_prim_learned(long n, const double *__restrict__ U,
@@ -694,8 +685,10 @@ struct maragpu_ctx {
   std::vector<hipEvent_t> ev;
   double flux_ms = 0.0, c2p_ms = 0.0, upd_ms = 0.0;
   int flux_n = 0, c2p_n = 0, upd_n = 0;
-  // learned c2p network (not owned; see maragpu_attach_mlp)
+  // learned c2p network (not owned; see maragpu_attach_mlp) and the Newton starts it
+  // writes (one (Z, W) per interior zone)
   const maragpu_mlp *mlp = nullptr;
+  double2 *start = nullptr;
 };
 
 namespace {
@@ -807,6 +800,9 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (hipMalloc(&c->dstats, sizeof(DevStats)) != hipSuccess) return fail_free(c);
   if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
   if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
+  if (cfg->c2p == MARAGPU_C2P_LEARNED &&
+      hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(double2)) != hipSuccess)
+    return fail_free(c);
   // fluxes outside the sweep ranges are never written; keep them defined
   for (int d = 0; d < 3; ++d)
     if (hipMemsetAsync(c->F[d], 0, fbytes, c->stream) != hipSuccess) return fail_free(c);
@@ -829,6 +825,7 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->dstats);
   if (c->hstats) hipHostFree(c->hstats);
   hipFree(c->ddiv);
+  hipFree(c->start);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -893,14 +890,15 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
       MlpDev net;
       if (maragpu_mlp_device_params(c->mlp, &net)) return MARAGPU_E_NOMODEL;
       const unsigned nb = nblocks(ni) < 4096u ? nblocks(ni) : 4096u;
-      k_cons_to_prim_learned<<<nb, kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
-                                                           first, c->cfg.ereset, c->cfg.preset,
-                                                           c->cfg.dreset, c->dstats, pass, net);
+      k_mlp_start<<<nb, kBlock, 0, c->stream>>>(c->U, g, c->cfg.gamma, net, c->start);
+      k_cons_to_prim<true><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+                                                                  first, c->cfg.ereset, c->cfg.preset,
+                                                                  c->cfg.dreset, c->dstats, pass, c->start);
     }
     else {
-      k_cons_to_prim<<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
-                                                            first, c->cfg.ereset, c->cfg.preset,
-                                                            c->cfg.dreset, c->dstats, pass);
+      k_cons_to_prim<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+                                                                   first, c->cfg.ereset, c->cfg.preset,
+                                                                   c->cfg.dreset, c->dstats, pass, nullptr);
     }
     HIP_CHECK(hipEventRecord(c->ev[p.b], c->stream));
     return 0;

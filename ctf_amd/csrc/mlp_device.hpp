@@ -65,10 +65,13 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f)
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-__device__ __forceinline__ float silu(float a) { return a / (1.0f + __expf(-a)); }
+/* SiLU a * sigmoid(a) and its derivative, with the hardware exponential and reciprocal
+ * (about 1 ulp of FP32 each): the results are rounded to bfloat16 before use. */
+__device__ __forceinline__ float sigmoid(float a) { return __builtin_amdgcn_rcpf(1.0f + __expf(-a)); }
+__device__ __forceinline__ float silu(float a) { return a * sigmoid(a); }
 __device__ __forceinline__ float silu_grad(float a)
 {
-  const float s = 1.0f / (1.0f + __expf(-a));
+  const float s = sigmoid(a);
   return s * (1.0f + a * (1.0f - s));
 }
 
@@ -119,13 +122,15 @@ __device__ __forceinline__ void mlp_features(const double *U, double gamma, floa
   const double S2 = U[2]*U[2] + U[3]*U[3] + U[4]*U[4];
   const double B2 = U[5]*U[5] + U[6]*U[6] + U[7]*U[7];
   const double SB = U[2]*U[5] + U[3]*U[6] + U[4]*U[7];
+  // ratios in FP64 (E - |S| needs it), logarithms in FP32: the features are FP32
   const double iD = 1.0 / D;
-  x[0] = (float)log(fmax(tau * iD, 1e-30));
-  x[1] = (float)log1p(S2 * iD * iD);
-  x[2] = (float)log1p(B2 * iD);
-  x[3] = (float)(SB / (sqrt(S2 * B2) + 1e-300));
+  x[0] = logf((float)fmax(tau * iD, 1e-30));
+  x[1] = log1pf((float)(S2 * iD * iD));
+  x[2] = log1pf((float)(B2 * iD));
+  const float q = (float)(S2 * B2);
+  x[3] = q > 0.0f ? (float)SB * rsqrtf(q) : 0.0f;
   const double E = tau + D, S = sqrt(S2);
-  x[4] = (float)(2.0 * log(E) - log(fmax((E - S) * (E + S), 1e-300)));
+  x[4] = logf((float)(E * E / fmax((E - S) * (E + S), 1e-300)));
   x[5] = (float)gamma;
 }
 

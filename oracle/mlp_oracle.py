@@ -125,9 +125,10 @@ def features(U, gamma):
     x[:, 0] = np.log(np.maximum(tau * iD, 1e-30))
     x[:, 1] = np.log1p(S2 * iD * iD)
     x[:, 2] = np.log1p(B2 * iD)
-    x[:, 3] = SB / (np.sqrt(S2 * B2) + 1e-300)
+    q = S2 * B2
+    x[:, 3] = np.where(q > 0, SB / np.sqrt(np.where(q > 0, q, 1.0)), 0.0)
     E = tau + D
     S = np.sqrt(S2)
-    x[:, 4] = 2.0 * np.log(E) - np.log(np.maximum((E - S) * (E + S), 1e-300))
+    x[:, 4] = np.log(E * E / np.maximum((E - S) * (E + S), 1e-300))
     x[:, 5] = gamma
     return x
