@@ -4,9 +4,14 @@
  * (re)built on first use after a change. Errors that the Lua binding reports with
  * luaL_error return MARA_E_* with a message in mara_last_error().
  */
+#include <dlfcn.h>
+
 #include <chrono>
 #include <cstring>
+#include <fstream>
+#include <iterator>
 #include <string>
+#include <vector>
 
 #include "../../include/mara_host.h"
 #include "../../include/maragpu.h"
@@ -23,6 +28,8 @@ struct App {
   bool have_advance = false;
   maragpu_config cfg;
   maragpu_ctx *ctx = nullptr;
+  maragpu_mlp *mlp = nullptr;      // learned c2p network, loaded on first use
+  std::string mlp_path;            // config_solver key 'c2p_model'; default models/c2p_mlp.bin
   bool dirty = true;
   double last_max_lambda = 0.0;
   std::string err;
@@ -71,6 +78,26 @@ void mark_dirty()
   if (g_app) g_app->dirty = true;
 }
 
+/* The network file: the 'c2p_model' path, else models/c2p_mlp.bin next to this library. */
+int load_mlp(App *a)
+{
+  std::string path = a->mlp_path;
+  if (path.empty()) {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&mara_start), &info) && info.dli_fname) {
+      std::string lib = info.dli_fname;
+      const size_t slash = lib.rfind('/');
+      path = (slash == std::string::npos ? std::string(".") : lib.substr(0, slash)) + "/models/c2p_mlp.bin";
+    }
+  }
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return fail("[Mara] c2p='learned' needs a network file; none at '" + path + "'", MARAGPU_E_NOMODEL);
+  std::vector<char> buf((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+  const int rc = maragpu_mlp_load(&a->mlp, buf.data(), buf.size());
+  if (rc) return fail("[Mara] cannot load the c2p network '" + path + "': " + maragpu_strerror(rc), rc);
+  return 0;
+}
+
 int ensure_ctx()
 {
   App *a = app();
@@ -87,6 +114,10 @@ int ensure_ctx()
   }
   int rc = maragpu_create(&a->cfg, &a->ctx);
   if (rc) return fail(std::string("[Mara] device context: ") + maragpu_strerror(rc), rc);
+  if (a->cfg.c2p == MARAGPU_C2P_LEARNED) {
+    if (!a->mlp && (rc = load_mlp(a))) return rc;
+    maragpu_attach_mlp(a->ctx, a->mlp);
+  }
   a->dirty = false;
   return 0;
 }
@@ -106,7 +137,7 @@ const char *mara_last_error(void) { return g_err.c_str(); }
 
 int mara_start(void)
 {
-  delete g_app;
+  mara_close();
   g_app = new App;
   g_app->started = true;
   return 0;
@@ -116,6 +147,7 @@ int mara_close(void)
 {
   if (g_app) {
     if (g_app->ctx) maragpu_destroy(g_app->ctx);
+    if (g_app->mlp) maragpu_mlp_destroy(g_app->mlp);
     delete g_app;
     g_app = nullptr;
   }
@@ -239,6 +271,15 @@ int mara_config_solver_str(const char *key, const char *value)
     if (!std::strcmp(value, "fp64")) c.precision = MARAGPU_FP64;
     else if (!std::strcmp(value, "mixed")) c.precision = MARAGPU_MIXED;
     else return fail(std::string("[Mara] no such precision '") + value + "'");
+    mark_dirty();
+    return 0;
+  }
+  if (std::strcmp(key, "c2p_model") == 0) {
+    if (app()->mlp) {
+      maragpu_mlp_destroy(app()->mlp);
+      app()->mlp = nullptr;
+    }
+    app()->mlp_path = value;
     mark_dirty();
     return 0;
   }

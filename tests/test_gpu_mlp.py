@@ -145,3 +145,45 @@ def test_save_and_load_round_trip(gpu, trained):
     th1, nm1 = net.params()
     th2, nm2 = net2.params()
     assert np.array_equal(th1, th2) and np.array_equal(nm1, nm2)
+
+
+def test_mara_verbs_select_the_learned_form(gpu, tmp_path):
+    """config_solver{c2p='learned'} loads the committed network next to the library and
+    gives the plain cascade's answer; a missing network file is an error."""
+    from ctf_amd.mara import Mara, MaraError
+    N = 16
+    P0 = gpu.init_turbulence((N, N, N), v_rms=0.3, seed=5)
+    out = {}
+    for c2p in ("plain", "learned"):
+        Mara.start()
+        Mara.set_fluid("rmhd")
+        Mara.set_eos("gamma-law", GAMMA)
+        Mara.set_advance("rk3")
+        Mara.set_godunov("plm-split")
+        Mara.set_boundary("periodic")
+        Mara.set_riemann("hlld")
+        Mara.config_solver({"extrap": "weno5", "c2p": c2p}, True)
+        Mara.set_domain((0, 0, 0), (1, 1, 1), (N, N, N), 8, 3)
+        P = P0.copy()
+        dt = 0.0
+        for _ in range(3):
+            _, ne = Mara.advance(P, dt)
+            assert ne == 0
+            dt = Mara.get_timestep(0.4)
+        out[c2p] = P
+        Mara.close()
+    it = (slice(3, -3),) * 3
+    a, b = out["learned"][it], out["plain"][it]
+    assert (np.abs(a - b).sum(axis=(0, 1, 2)) / np.abs(b).sum(axis=(0, 1, 2))).max() < 1e-10
+    Mara.start()
+    Mara.set_fluid("rmhd")
+    Mara.set_eos("gamma-law", GAMMA)
+    Mara.set_advance("rk2")
+    Mara.set_godunov("plm-split")
+    Mara.set_boundary("periodic")
+    Mara.set_riemann("hlld")
+    Mara.config_solver({"c2p": "learned", "c2p_model": str(tmp_path / "none.bin")}, True)
+    Mara.set_domain((0, 0, 0), (1, 1, 1), (N, N, N), 8, 3)
+    with pytest.raises(MaraError):
+        Mara.advance(P0.copy(), 0.0)
+    Mara.close()
