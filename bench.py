@@ -271,6 +271,9 @@ def main():
         rec["c2p_fallback_fraction"] = st.get("c2p_fallbacks", 0) / zones
         rec["hlld_to_hllc_fallbacks"] = st.get("hlld_failure", 0)
         rec["hlld_solves"] = st.get("hlld_success", 0) + st.get("hlld_failure", 0)
+        if st.get("fp64_redo", 0) or st.get("fp64_redo_overflow", 0):
+            rec["fp64_redo_interfaces"] = st.get("fp64_redo", 0)
+            rec["fp64_redo_overflow"] = st.get("fp64_redo_overflow", 0)
         if name == "learned":
             g = max(st.get("mlp_guesses", 0), 1)
             rec["mlp_fallback_fraction"] = st.get("mlp_fallbacks", 0) / g

@@ -24,6 +24,9 @@ HIPCC_FLAGS = [
     "-Wno-unused-value", "-Wno-unused-result",
     # packed FP32 instructions off (see DESIGN: hardware trap on this pool)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # matrix-core accumulators in ordinary VGPRs: the network's activations are read by
+    # VALU right after each product, which the accumulator-register form makes a copy
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 

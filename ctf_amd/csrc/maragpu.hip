@@ -55,7 +55,17 @@ struct DevStats {
   unsigned long long hlld_newton_iters;
   unsigned long long mlp_guesses;
   unsigned long long mlp_fallbacks;
+  unsigned long long redo;             // mixed form: interfaces recomputed in FP64
+  unsigned long long redo_overflow;    // ... that did not fit the queue (FP32 HLLC kept)
   unsigned long long stage_failed[6];  // failed zones per c2p pass (ghost images included)
+};
+
+/* Mixed form: interfaces whose FP32 HLLD solve failed, recomputed in FP64 by
+ * k_flux_redo before the update reads the fluxes. */
+struct RedoQueue {
+  unsigned int *count;     // device counter, cleared before each sweep
+  unsigned int cap;        // slots
+  long *z;                 // device slots: interface index z (left zone)
 };
 
 struct GridDesc {
@@ -202,7 +212,7 @@ __device__ __forceinline__ int scatter_images(const GridDesc &g, double *__restr
  * from the network's Newton start (k_mlp_start); a zone whose solve fails runs the plain
  * cascade. */
 template <bool LEARNED>
-__global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restrict__ U,
+__global__ __launch_bounds__(kBlock, 5) void k_cons_to_prim(const double *__restrict__ U,
                                                          double *__restrict__ P, GridDesc g,
                                                          double gamma, double pfloor, int first,
                                                          int ereset, double preset, double dreset,
@@ -221,12 +231,9 @@ __global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restric
     const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
     double u[8], p[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      u[q] = U[q * T + z];
-      p[q] = first ? 0.0 : P[q * T + z];
-    }
+    for (int q = 0; q < 8; ++q) u[q] = U[q * T + z];
     int iters = 0, which = 0, err = 1, used = 0;
-    if (LEARNED) {
+    if constexpr (LEARNED) {
       err = check_cons(u);
       if (!err) {
         C2PState s;
@@ -236,7 +243,12 @@ __global__ __launch_bounds__(kBlock) void k_cons_to_prim(const double *__restric
         used = err == 0;
       }
     }
-    if (err) err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
+    if (err) {
+      // the previous primitive state (the cascade's second guess) is read only here
+#pragma unroll
+      for (int q = 0; q < 8; ++q) p[q] = first ? 0.0 : P[q * T + z];
+      err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
+    }
     int failed = err != 0;
     if (failed && ereset) {
       p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
@@ -283,12 +295,12 @@ __device__ __forceinline__ void learned_start(double D, float y0, float y1, cons
 /* The learned c2p's network pass over the interior: one wave per 64 zones on the matrix
  * cores, writing each zone's Newton start (Z, W) (interior index order). Blocks loop over
  * 256-zone chunks so the weights are staged into LDS once per block. */
-__global__ __launch_bounds__(kBlock) void k_mlp_start(const double *__restrict__ U, GridDesc g, double gamma,
+__global__ __launch_bounds__(kBlock, 4) void k_mlp_start(const double *__restrict__ U, GridDesc g, double gamma,
                                                       MlpDev net, double2 *__restrict__ start)
 {
-  __shared__ uint16_t wf[MLP_NFWD * MLP_FRAG];
-  __shared__ float bias[3 * 64 + 2];
-  __shared__ uint16_t xin[kBlock / 64][64 * MLP_KIN];
+  __shared__ __align__(16) uint16_t wf[MLP_NFWD * MLP_FRAG];
+  __shared__ __align__(16) float bias[3 * 64 + 2];
+  __shared__ __align__(16) uint16_t xin[kBlock / 64][64 * MLP_KIN];
   __shared__ float yout[kBlock / 64][128];
   for (int i = threadIdx.x; i < MLP_NFWD * MLP_FRAG / 4; i += kBlock)
     reinterpret_cast<uint2 *>(wf)[i] = reinterpret_cast<const uint2 *>(net.fwd)[i];
@@ -335,7 +347,7 @@ __global__ __launch_bounds__(kBlock) void kb_cons_to_prim_learned(long n, const 
                                                                   double pfloor, int *err_out, int *iters_out,
                                                                   int *used_out, double *start_out, MlpDev net)
 {
-  __shared__ uint16_t xin[kBlock / 64][64 * MLP_KIN];
+  __shared__ __align__(16) uint16_t xin[kBlock / 64][64 * MLP_KIN];
   __shared__ float yout[kBlock / 64][128];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long t = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -392,71 +404,144 @@ __device__ __forceinline__ int riemann_dispatch(int solver, const R *pl, const R
   return riemann_hll<R, DIM>(pl, pr, F, gamma, lam);
 }
 
+/* One interface z | z+S of the sweep along DIM: reconstruction from the 6-point stencil,
+ * the Riemann solve, and the retry with the zone-centred states of the two neighbours
+ * (Mara/plm-split.cpp:51-71). Returns the error of the last attempt (flux 0 then). */
+template <typename R, int DIM, int SOLVER>
+__device__ __forceinline__ int interface_flux(const double *__restrict__ P, long T, long S, long z, int recon,
+                                              const PhysParams &pp, R *F, R &lam, FluxCounters &fc,
+                                              int &retried)
+{
+  R ql[8], qr[8];
+  const R theta = (R)pp.theta;
+  for (int q = 0; q < 8; ++q) {
+    const double *Pq = P + q * T + z;
+    const R v0 = (R)Pq[0], v1 = (R)Pq[S];
+    if (recon == RECON_WENO5) {
+      const R vm2 = (R)Pq[-2 * S], vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S], v3 = (R)Pq[3 * S];
+      ql[q] = weno5<R, true>(vm2, vm1, v0, v1, v2, pp.IS, (R)pp.sz10A);
+      qr[q] = weno5<R, false>(vm1, v0, v1, v2, v3, pp.IS, (R)pp.sz10A);
+    }
+    else if (recon == RECON_PLM) {
+      const R vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S];
+      ql[q] = plm<R>(vm1, v0, v1, R(+1.0), theta);
+      qr[q] = plm<R>(v0, v1, v2, R(-1.0), theta);
+    }
+    else {
+      ql[q] = v0;
+      qr[q] = v1;
+    }
+  }
+  int err = 1;
+  retried = 0;
+  // one call site: the reconstructed states, then the zone-centred states
+#pragma unroll Shape<R>::unroll
+  for (int attempt = 0; attempt < 2 && err; ++attempt) {
+    if (attempt == 1) {
+      retried = 1;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        ql[q] = (R)P[q * T + z];
+        qr[q] = (R)P[q * T + z + S];
+      }
+    }
+    err = riemann_dispatch<R, DIM>(SOLVER, ql, qr, F, (R)pp.gamma, pp.pfloor, lam, fc);
+  }
+  if (err) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) F[q] = R(0);
+  }
+  return err;
+}
+
+/* Flux sweep along DIM, one thread per interface. With a redo queue (mixed form, HLLD),
+ * an interface whose FP32 HLLD solve fell back to HLLC or failed is queued for
+ * k_flux_redo instead; its FP32 result stays only if the queue is full. */
+/* Occupancy targets (waves per SIMD), measured at 256^3 WENO5/HLLD: FP32 sweeps run
+ * best at 3 (168 VGPRs, little spilling; 3.06 -> 2.26 ms), FP64 sweeps at 2 even though
+ * the register budget of 256 then spills (7.2 -> 6.2 ms: the spill traffic costs less
+ * than the latency that a single wave cannot hide). */
 template <typename R, typename FT, int DIM, int SOLVER>
-__global__ __launch_bounds__(kBlock) void k_flux(const double *__restrict__ P, FT *__restrict__ Fout,
-                                                 GridDesc g, int recon, PhysParams pp, DevStats *st)
+__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 2) void k_flux(const double *__restrict__ P, FT *__restrict__ Fout,
+                                                 GridDesc g, int recon, PhysParams pp, DevStats *st,
+                                                 RedoQueue rq)
 {
   const long T = g.T;
   const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
   const long z = 2 * S + (long)blockIdx.x * blockDim.x + threadIdx.x;
   double lam_out = 0.0;
-  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long cnt[7] = {0, 0, 0, 0, 0, 0, 0};
   if (z < T - 3 * S) {
-    R ql[8], qr[8];
-    const R theta = (R)pp.theta;
-    for (int q = 0; q < 8; ++q) {
-      const double *Pq = P + q * T + z;
-      const R v0 = (R)Pq[0], v1 = (R)Pq[S];
-      if (recon == RECON_WENO5) {
-        const R vm2 = (R)Pq[-2 * S], vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S], v3 = (R)Pq[3 * S];
-        ql[q] = weno5<R, true>(vm2, vm1, v0, v1, v2, pp.IS, (R)pp.sz10A);
-        qr[q] = weno5<R, false>(vm1, v0, v1, v2, v3, pp.IS, (R)pp.sz10A);
-      }
-      else if (recon == RECON_PLM) {
-        const R vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S];
-        ql[q] = plm<R>(vm1, v0, v1, R(+1.0), theta);
-        qr[q] = plm<R>(v0, v1, v2, R(-1.0), theta);
-      }
-      else {
-        ql[q] = v0;
-        qr[q] = v1;
-      }
-    }
     R lam = R(0);
     FluxCounters fc = {0, 0, 0, 0};
     R F[8];
-    int err = 1;
-    // one call site: the reconstructed states, then (Mara/plm-split.cpp:68-71) the
-    // zone-centred states of the two neighbours
-#pragma unroll Shape<R>::unroll
-    for (int attempt = 0; attempt < 2 && err; ++attempt) {
-      if (attempt == 1) {
-        cnt[2] = 1;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          ql[q] = (R)P[q * T + z];
-          qr[q] = (R)P[q * T + z + S];
-        }
+    int retried;
+    const int err = interface_flux<R, DIM, SOLVER>(P, T, S, z, recon, pp, F, lam, fc, retried);
+    bool queued = false;
+    if (rq.z && (fc.hlld_fail || err)) {
+      const unsigned int k = atomicAdd(rq.count, 1u);
+      if (k < rq.cap) {
+        rq.z[k] = z;
+        queued = true;
       }
-      err = riemann_dispatch<R, DIM>(SOLVER, ql, qr, F, (R)pp.gamma, pp.pfloor, lam, fc);
-    }
-    if (err) {
-      cnt[3] = 1;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) F[q] = R(0);
+      else cnt[6] = 1;
     }
 #pragma unroll
     for (int q = 0; q < 8; ++q) Fout[q * T + z] = (FT)F[q];
     lam_out = (double)lam;
-    cnt[0] = (unsigned long long)fc.hlld_ok;
-    cnt[1] = (unsigned long long)fc.hlld_fail;
+    if (!queued) {
+      cnt[0] = (unsigned long long)fc.hlld_ok;
+      cnt[1] = (unsigned long long)fc.hlld_fail;
+      cnt[2] = (unsigned long long)retried;
+      cnt[3] = err ? 1ull : 0ull;
+    }
     cnt[4] = (unsigned long long)fc.c2p_iters;
     cnt[5] = (unsigned long long)fc.newton_iters;
   }
   block_max_lambda(lam_out, &st->lam_bits);
-  unsigned long long *const dst[6] = { &st->hlld_success, &st->hlld_failure, &st->retries,
-                                       &st->failures, &st->hlld_c2p_iters, &st->hlld_newton_iters };
-  block_add<6>(cnt, dst);
+  unsigned long long *const dst[7] = { &st->hlld_success, &st->hlld_failure, &st->retries,
+                                       &st->failures, &st->hlld_c2p_iters, &st->hlld_newton_iters,
+                                       &st->redo_overflow };
+  block_add<7>(cnt, dst);
+}
+
+/* The queued interfaces of a mixed-form sweep, recomputed as the plain FP64 method does
+ * (reconstruction, HLLD attempt ladder, HLLC fallback, retry) and written over the FP32
+ * fluxes. One thread per queue slot; slots past the count exit. */
+template <int DIM, int SOLVER>
+__global__ __launch_bounds__(kBlock) void k_flux_redo(const double *__restrict__ P, float *__restrict__ Fout,
+                                                      GridDesc g, int recon, PhysParams pp, DevStats *st,
+                                                      RedoQueue rq)
+{
+  const unsigned int n = *rq.count;
+  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double lam_out = 0.0;
+  unsigned long long cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  if (k < n && k < rq.cap) {
+    const long T = g.T;
+    const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+    const long z = rq.z[k];
+    double lam = 0.0;
+    FluxCounters fc = {0, 0, 0, 0};
+    double F[8];
+    int retried;
+    const int err = interface_flux<double, DIM, SOLVER>(P, T, S, z, recon, pp, F, lam, fc, retried);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Fout[q * T + z] = (float)F[q];
+    lam_out = lam;
+    cnt[0] = (unsigned long long)fc.hlld_ok;
+    cnt[1] = (unsigned long long)fc.hlld_fail;
+    cnt[2] = (unsigned long long)retried;
+    cnt[3] = err ? 1ull : 0ull;
+    cnt[4] = (unsigned long long)fc.c2p_iters;
+    cnt[5] = (unsigned long long)fc.newton_iters;
+    cnt[6] = 1;
+  }
+  block_max_lambda(lam_out, &st->lam_bits);
+  unsigned long long *const dst[7] = { &st->hlld_success, &st->hlld_failure, &st->retries,
+                                       &st->failures, &st->hlld_c2p_iters, &st->hlld_newton_iters,
+                                       &st->redo };
+  block_add<7>(cnt, dst);
 }
 
 /* --------------------------------------------------------------------------
@@ -689,6 +774,8 @@ struct maragpu_ctx {
   // writes (one (Z, W) per interior zone)
   const maragpu_mlp *mlp = nullptr;
   double2 *start = nullptr;
+  // mixed form with HLLD: the FP64 redo queue of each sweep
+  RedoQueue redo = { nullptr, 0u, nullptr };
 };
 
 namespace {
@@ -735,15 +822,25 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
   const long n = c->g.T - 5 * S;
   if (n <= 0) return;
   FT *F = static_cast<FT *>(c->F[DIM - 1]);
+  RedoQueue none = { nullptr, 0u, nullptr };
   switch (c->cfg.riemann) {
   case MARAGPU_HLL:
-    k_flux<R, FT, DIM, RIEMANN_HLL><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats);
+    k_flux<R, FT, DIM, RIEMANN_HLL><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, none);
     break;
   case MARAGPU_HLLC:
-    k_flux<R, FT, DIM, RIEMANN_HLLC><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats);
+    k_flux<R, FT, DIM, RIEMANN_HLLC><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, none);
     break;
   default:
-    k_flux<R, FT, DIM, RIEMANN_HLLD><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats);
+    if constexpr (sizeof(R) == 4) {
+      // mixed form: FP32 sweep, then the interfaces whose FP32 HLLD failed redone in FP64
+      hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
+      k_flux<R, FT, DIM, RIEMANN_HLLD><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, c->redo);
+      k_flux_redo<DIM, RIEMANN_HLLD><<<(c->redo.cap + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
+          c->Pw, reinterpret_cast<float *>(F), c->g, c->cfg.recon, pp, c->dstats, c->redo);
+    }
+    else {
+      k_flux<R, FT, DIM, RIEMANN_HLLD><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, none);
+    }
     break;
   }
 }
@@ -803,6 +900,14 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (cfg->c2p == MARAGPU_C2P_LEARNED &&
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(double2)) != hipSuccess)
     return fail_free(c);
+  if (cfg->precision == MARAGPU_MIXED && cfg->riemann == MARAGPU_HLLD) {
+    // room for 1 interface in 256 (FP32 HLLD fails on about 1 in 10^5 in turbulence)
+    const long cap = c->g.T / 256 > 4096 ? c->g.T / 256 : 4096;
+    c->redo.cap = (unsigned int)cap;
+    if (hipMalloc(&c->redo.count, sizeof(unsigned int)) != hipSuccess ||
+        hipMalloc(&c->redo.z, (size_t)cap * sizeof(long)) != hipSuccess)
+      return fail_free(c);
+  }
   // fluxes outside the sweep ranges are never written; keep them defined
   for (int d = 0; d < 3; ++d)
     if (hipMemsetAsync(c->F[d], 0, fbytes, c->stream) != hipSuccess) return fail_free(c);
@@ -826,6 +931,8 @@ int maragpu_destroy(maragpu_ctx *c)
   if (c->hstats) hipHostFree(c->hstats);
   hipFree(c->ddiv);
   hipFree(c->start);
+  hipFree(c->redo.count);
+  hipFree(c->redo.z);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -992,6 +1099,8 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     stats->hlld_newton_iterations = (int64_t)h.hlld_newton_iters;
     stats->mlp_guesses = (int64_t)h.mlp_guesses;
     stats->mlp_fallbacks = (int64_t)h.mlp_fallbacks;
+    stats->fp64_redo = (int64_t)h.redo;
+    stats->fp64_redo_overflow = (int64_t)h.redo_overflow;
     float ms = 0.f;
     hipEventElapsedTime(&ms, c->ev[ev_start], c->ev[ev_end]);
     stats->device_ms = ms;

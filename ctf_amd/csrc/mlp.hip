@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void k_train_fwd(const float *__restrict__ X, 
                                                    uint16_t *__restrict__ H, float *__restrict__ act,
                                                    float *__restrict__ yout, float *__restrict__ tout)
 {
-  __shared__ uint16_t xin[4][64 * MLP_KIN];
+  __shared__ __align__(16) uint16_t xin[4][64 * MLP_KIN];
   __shared__ float ybuf[4][128];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long s = (long)blockIdx.x * 256 + threadIdx.x;        // sample within the batch
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void k_infer(const float *__restrict__ Xs, lon
                                                const uint16_t *__restrict__ fwd, const float *__restrict__ th,
                                                float *__restrict__ Y)
 {
-  __shared__ uint16_t xin[4][64 * MLP_KIN];
+  __shared__ __align__(16) uint16_t xin[4][64 * MLP_KIN];
   __shared__ float ybuf[4][128];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long s = (long)blockIdx.x * 256 + threadIdx.x;

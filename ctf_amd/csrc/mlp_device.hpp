@@ -55,14 +55,18 @@ struct MlpDev {
   float ymean[MLP_OUT], ystd[MLP_OUT];
 };
 
-__device__ __forceinline__ uint16_t f32_to_bf16(float f)
+typedef __bf16 bf16x2v_t __attribute__((ext_vector_type(2)));
+typedef float f32x2v_t __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+
+/* FP32 -> bfloat16, round to nearest even, with gfx950's v_cvt_pk_bf16_f32 (two values
+ * per instruction). */
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b)
 {
-  // round to nearest even (finite inputs; NaN stays NaN)
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  const f32x2v_t v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2v_t));
 }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 /* SiLU a * sigmoid(a) and its derivative, with the hardware exponential and reciprocal
@@ -79,14 +83,19 @@ __device__ __forceinline__ float silu_grad(float a)
  * hi/lo bfloat16 pairs in columns k and MLP_IN + k, zero padding after 2 MLP_IN. */
 __device__ __forceinline__ void mlp_input_row(const float *xs, uint16_t *row)
 {
+  uint32_t *w = reinterpret_cast<uint32_t *>(row);     // row is 4-byte aligned (MLP_KIN even)
+  float lo[MLP_IN];
 #pragma unroll
-  for (int k = 0; k < MLP_IN; ++k) {
-    const uint16_t hi = f32_to_bf16(xs[k]);
-    row[k] = hi;
-    row[MLP_IN + k] = f32_to_bf16(xs[k] - bf16_to_f32(hi));
+  for (int k = 0; k < MLP_IN; k += 2) {
+    const uint32_t hi = pack_bf16x2(xs[k], xs[k + 1]);
+    w[k / 2] = hi;
+    lo[k] = xs[k] - __uint_as_float(hi << 16);
+    lo[k + 1] = xs[k + 1] - __uint_as_float(hi & 0xffff0000u);
   }
 #pragma unroll
-  for (int k = 2 * MLP_IN; k < MLP_KIN; ++k) row[k] = 0;
+  for (int k = 0; k < MLP_IN; k += 2) w[(MLP_IN + k) / 2] = pack_bf16x2(lo[k], lo[k + 1]);
+#pragma unroll
+  for (int k = 2 * MLP_IN; k < MLP_KIN; k += 2) w[k / 2] = 0u;
 }
 
 __device__ __forceinline__ f32x4_t mfma16(bf16x4_t a, bf16x4_t b, f32x4_t c)
@@ -107,12 +116,8 @@ __device__ __forceinline__ bf16x4_t load_frag(const uint16_t *frags, int f, int 
 
 __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d)
 {
-  bf16x4_t r;
-  r[0] = (short)f32_to_bf16(a);
-  r[1] = (short)f32_to_bf16(b);
-  r[2] = (short)f32_to_bf16(c);
-  r[3] = (short)f32_to_bf16(d);
-  return r;
+  const u32x2_t v = {pack_bf16x2(a, b), pack_bf16x2(c, d)};
+  return __builtin_bit_cast(bf16x4_t, v);
 }
 
 /* Features of one conserved state (FP32), before standardisation. */
@@ -157,61 +162,61 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t *xin, const uint
     X[nt][2] = (short)(v.y & 0xffff);
     X[nt][3] = (short)(v.y >> 16);
   }
-  // layer 1: 64 x 16 . 16 x 64
+  // layer 1: 64 x 16 . 16 x 64; accumulators start at the bias
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const bf16x4_t a = load_frag(wf, MLP_F1 + mt, lane);
-    float bias[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) bias[r] = b1[16 * mt + 4 * grp + r];
+    const f32x4_t bias = *reinterpret_cast<const f32x4_t *>(b1 + 16 * mt + 4 * grp);
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
-      f32x4_t c = {0.f, 0.f, 0.f, 0.f};
-      c = mfma16(a, X[nt], c);
+      const f32x4_t c = mfma16(a, X[nt], bias);
       float h[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const float z = c[r] + bias[r];
-        if (act) act[(0 * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = z;
-        h[r] = silu(z);
+        if (act) act[(0 * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = c[r];
+        h[r] = silu(c[r]);
       }
       H[mt][nt] = pack4(h[0], h[1], h[2], h[3]);
     }
   }
-  // layers 2 and 3: 64 x 64 . 64 x 64
+  // layers 2 and 3: 64 x 64 . 64 x 64, in two halves of 32 samples: a half's new
+  // activations replace its old ones only after all of its products are formed, which
+  // keeps 32 accumulators live instead of 64
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
     const int f0 = layer == 0 ? MLP_F2 : MLP_F3;
     const float *bb = layer == 0 ? b2 : b3;
-    f32x4_t C[4][4];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
+    for (int half = 0; half < 2; ++half) {
+      f32x4_t C[4][2];
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) C[mt][nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-#pragma unroll
-      for (int kt = 0; kt < 4; ++kt) {
-        const bf16x4_t a = load_frag(wf, f0 + 4 * mt + kt, lane);
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) C[mt][nt] = mfma16(a, H[kt][nt], C[mt][nt]);
+      for (int mt = 0; mt < 4; ++mt) {
+        const f32x4_t bias = *reinterpret_cast<const f32x4_t *>(bb + 16 * mt + 4 * grp);
+        C[mt][0] = bias;
+        C[mt][1] = bias;
       }
-    }
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-      float bias[4];
+      for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) bias[r] = bb[16 * mt + 4 * grp + r];
+        for (int kt = 0; kt < 4; ++kt) {
+          const bf16x4_t a = load_frag(wf, f0 + 4 * mt + kt, lane);
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        float h[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float z = C[mt][nt][r] + bias[r];
-          if (act) act[((layer + 1) * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = z;
-          h[r] = silu(z);
+          for (int j = 0; j < 2; ++j) C[mt][j] = mfma16(a, H[kt][2 * half + j], C[mt][j]);
         }
-        H[mt][nt] = pack4(h[0], h[1], h[2], h[3]);
+      }
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int nt = 2 * half + j;
+          float h[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (act) act[((layer + 1) * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = C[mt][j][r];
+            h[r] = silu(C[mt][j][r]);
+          }
+          H[mt][nt] = pack4(h[0], h[1], h[2], h[3]);
+        }
       }
     }
   }

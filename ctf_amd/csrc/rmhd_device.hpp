@@ -34,6 +34,7 @@ enum { RIEMANN_HLL = 0, RIEMANN_HLLC = 1, RIEMANN_HLLD = 2 };
  * FP32 scales the ones that sit below single-precision resolution. */
 template <typename R> struct Tol;
 template <> struct Tol<double> {
+  static constexpr bool nested_c2p_from_average = false;   // the reference's cascade order
   static constexpr double newton_ftol = 1e-12;   // NewtonRaphesonSolver(15, 1e-12), riemann_hlld-rmhd.cpp:220
   static constexpr double fd_dp = 1e-8;          // HlldEquation48::Jacobian, :313
   static constexpr bool fd_relative = false;
@@ -49,8 +50,9 @@ template <> struct Tol<double> {
   static constexpr double hllc_small_a = 1e-10;  // riemann_hllc-rmhd.cpp:23
 };
 template <> struct Tol<float> {
+  static constexpr bool nested_c2p_from_average = true;
   static constexpr float newton_ftol = 2e-6f;
-  static constexpr float fd_dp = 1e-3f;          // centred difference, relative to max(|p|, 1e-3)
+  static constexpr float fd_dp = 1e-4f;          // centred difference, relative to max(|p|, 1e-3)
   static constexpr bool fd_relative = true;
   static constexpr float X_floor = 1e-8f;
   static constexpr float small_delK = 1e-6f;
@@ -249,6 +251,32 @@ template <typename R> __device__ __forceinline__ void quartic_extremes(R d4, R d
   }
 }
 
+/* Extreme root of the characteristic quartic of FluxAndEigenvalues in its factored form
+ *     Q(x) = K (x - v)^4 + (1 - x^2) [L (x - v)^2 + (b0 x - bi)^2],
+ * which is the reference's A4 x^4 + ... + A0 (Mara/rmhd.cpp:240-258) before expansion.
+ * All four roots are real and Q has a positive leading coefficient, so Newton's method
+ * started outside the roots moves monotonically onto the extreme one. Evaluated in this
+ * form, Q keeps its relative accuracy near the roots even when they crowd within 1/W^2 of
+ * the flow speed, which is where single-precision Ferrari loses every digit. */
+template <typename R>
+__device__ __forceinline__ R quartic_root_from_outside(R K, R L, R b0, R bi, R v, R x)
+{
+#pragma unroll 1
+  for (int it = 0; it < 24; ++it) {
+    const R y = x - v, y2 = y*y, c = b0*x - bi;
+    const R q1 = L*y2 + c*c;
+    const R om = (R(1) - x)*(R(1) + x);
+    const R Q = K*y2*y2 + om*q1;
+    const R dQ = R(4)*K*y2*y - R(2)*x*q1 + om*(R(2)*L*y + R(2)*b0*c);
+    const R step = Q * rcp(dQ);
+    const R xn = x - step;
+    if (!(dfabs(xn) <= R(1))) break;      // not finite, or out of the light cone: keep x
+    x = xn;
+    if (!(dfabs(step) > R(1e-7))) break;
+  }
+  return x;
+}
+
 /* Rmhd::FluxAndEigenvalues, Mara/rmhd.cpp:164-265, for normal axis DIM (1, 2, 3).
  * NEED_EIG = false skips the eigenvalues. */
 template <typename R, int DIM, bool NEED_EIG = true>
@@ -303,11 +331,31 @@ __device__ __forceinline__ void flux_and_eigenvalues(const R *U, const R *P, R *
   const R A2 = R(6)*K*v2 + L*(R(1.0)-v2) + b0*b0 - bi*bi;
   const R A1 = R(-4)*K*v3 - L*vi*R(2) - R(2)*b0*bi;
   const R A0 = K*v4 + L*v2 + bi*bi;
-  R rmax, rmin;
-  int nr;
-  quartic_extremes(A4, A3, A2, A1, A0, rmax, rmin, nr);
-  ap = rmax;
-  am = rmin;
+  if constexpr (sizeof(R) == 4) {
+    // FP32 (mixed form): Newton on the factored quartic, started from the lab-frame
+    // speeds of an isotropic wave at the largest comoving fast speed,
+    // w^2 = cs^2 + ca^2 - cs^2 ca^2 (perpendicular propagation), which bound the extreme
+    // roots from outside
+    const R cs2 = gamma*P[pre]*rcp(rhoh);
+    const R ca2 = b2*rcp(rhoh + b2);
+    const R w2 = cs2 + ca2 - cs2*ca2;
+    const R den = rcp(R(1) - V2*w2);
+    const R rt = dsqrt(w2*(R(1) - V2)*(R(1) - V2*w2 - vi*vi*(R(1) - w2)));
+    R lp = (vi*(R(1) - w2) + rt)*den + R(4e-6);
+    R lm = (vi*(R(1) - w2) - rt)*den - R(4e-6);
+    lp = lp < R(1) ? lp : R(1);
+    lm = lm > R(-1) ? lm : R(-1);
+    ap = quartic_root_from_outside(K, L, b0, bi, vi, lp);
+    am = quartic_root_from_outside(K, L, b0, bi, vi, lm);
+    (void)A4; (void)A3; (void)A2; (void)A1; (void)A0; (void)v3; (void)v4;
+  }
+  else {
+    R rmax, rmin;
+    int nr;
+    quartic_extremes(A4, A3, A2, A1, A0, rmax, rmin, nr);
+    ap = rmax;
+    am = rmin;
+  }
   if (dfabs(ap) > R(1.0) || dfabs(am) > R(1.0)) {
     ap = R(1.0);
     am = R(-1.0);
@@ -1097,8 +1145,26 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
     e.Lm2 = am*Ul[Sz] - Fl[Sz]; e.Rm2 = ap*Ur[Sz] - Fr[Sz];
   }
   int c2p_first;
-  const int hll_c2p_failed = cons_to_prim<Shape<R>::unroll>(Uh, Ph, (double)gamma, pfloor, true,
-                                                              cnt.c2p_iters, c2p_first);
+  int hll_c2p_failed = 1;
+  if constexpr (T::nested_c2p_from_average) {
+    // mixed form: the HLL state lies between the two reconstructed states, so anton2dzw
+    // starts from their average (the cascade's second guess) and the cascade runs only
+    // if that fails; the result seeds the FP32 p* Newton solve
+    if (!check_cons(Uh)) {
+      C2PState s;
+      c2p_new_state(s, Uh, (double)gamma, pfloor);
+      double Z, W;
+      c2p_starting_prim(s, Ph, Z, W);
+      double Pg[8];
+      hll_c2p_failed = c2p_anton2dzw(s, Z, W, Pg, cnt.c2p_iters);
+      if (!hll_c2p_failed) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) Ph[i] = Pg[i];
+      }
+    }
+  }
+  if (hll_c2p_failed)
+    hll_c2p_failed = cons_to_prim<Shape<R>::unroll>(Uh, Ph, (double)gamma, pfloor, true, cnt.c2p_iters, c2p_first);
 
   bool success = false;
   R Fx[8];

@@ -66,7 +66,8 @@ class Stats(C.Structure):
                 ("c2p_iterations", C.c_int64), ("c2p_fallbacks", C.c_int64),
                 ("c2p_failed", C.c_int64), ("hlld_c2p_iterations", C.c_int64),
                 ("hlld_newton_iterations", C.c_int64), ("mlp_guesses", C.c_int64),
-                ("mlp_fallbacks", C.c_int64), ("device_ms", C.c_double)]
+                ("mlp_fallbacks", C.c_int64), ("fp64_redo", C.c_int64),
+                ("fp64_redo_overflow", C.c_int64), ("device_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

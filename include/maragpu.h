@@ -82,6 +82,8 @@ typedef struct maragpu_stats {
   int64_t hlld_newton_iterations; /* p* Newton iterations in HLLD */
   int64_t mlp_guesses;          /* learned c2p: zones started from the network's guess */
   int64_t mlp_fallbacks;        /* learned c2p: zones that fell back to the plain cascade */
+  int64_t fp64_redo;            /* mixed form: interfaces whose FP32 HLLD failed, redone in FP64 */
+  int64_t fp64_redo_overflow;   /* ... beyond the redo queue (FP32 HLLC flux kept) */
   double device_ms;             /* device time of the step */
 } maragpu_stats;
 
