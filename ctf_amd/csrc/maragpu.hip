@@ -40,6 +40,8 @@ using namespace maragpu;
 namespace {
 
 constexpr int kBlock = 256;
+// learned c2p: network and polish in one kernel (true) or as two passes over the grid
+constexpr bool kFusedLearnedC2p = true;
 
 /* Device-side counters of one step; reduced per block, then atomically. */
 struct DevStats {
@@ -290,6 +292,94 @@ __device__ __forceinline__ void learned_start(double D, float y0, float y1, cons
   const double hg = 1.0 + gamma / (gamma - 1.0) * pg / rg;
   Zg = rg * hg * W * W;
   Wg = W;
+}
+
+/* The learned c2p in one pass: per 256-zone chunk, the network guess (one wave per 64
+ * zones on the matrix cores, weights staged in LDS once per block), the FP64 anton2dzw
+ * polish from it, and the plain cascade for zones whose polish fails. Waves of a CU
+ * interleave the network's VALU/MFMA work with other waves' FP64 Newton latency, and U
+ * is read once: 1.62 ms per launch at 256^3 against 1.98 for the two-pass form. Two
+ * waves per SIMD keep it free of spills (193 VGPRs). */
+__global__ __launch_bounds__(kBlock, 2) void k_cons_to_prim_fused(const double *__restrict__ U,
+                                                                  double *__restrict__ P, GridDesc g,
+                                                                  double gamma, double pfloor, int first,
+                                                                  int ereset, double preset, double dreset,
+                                                                  DevStats *st, int pass, MlpDev net)
+{
+  __shared__ __align__(16) uint16_t wf[MLP_NFWD * MLP_FRAG];
+  __shared__ __align__(16) float bias[3 * 64 + 2];
+  __shared__ __align__(16) uint16_t xin[kBlock / 64][64 * MLP_KIN];
+  __shared__ float yout[kBlock / 64][128];
+  for (int i = threadIdx.x; i < MLP_NFWD * MLP_FRAG / 4; i += kBlock)
+    reinterpret_cast<uint2 *>(wf)[i] = reinterpret_cast<const uint2 *>(net.fwd)[i];
+  for (int i = threadIdx.x; i < 3 * 64 + 2; i += kBlock)
+    bias[i] = i < 64 ? net.b1[i] : i < 128 ? net.b2[i - 64] : i < 192 ? net.b3[i - 128] : net.b4[i - 192];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  const long T = g.T;
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  for (long base = (long)blockIdx.x * kBlock; base < ni; base += (long)gridDim.x * kBlock) {
+    const long t = base + threadIdx.x;
+    const bool valid = t < ni;
+    int i = 0, j = 0, k = 0;
+    long z = 0;
+    double u[8];
+    if (valid) {
+      k = (int)(t % g.N[2]) + g.Ng;
+      j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+      i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+      z = ((long)i * g.n[1] + j) * g.n[2] + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) u[q] = U[q * T + z];
+    }
+    else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) u[q] = q < 2 ? 1.0 : 0.0;
+    }
+    float x[MLP_IN];
+    mlp_features(u, gamma, x);
+#pragma unroll
+    for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) / net.xstd[q];
+    mlp_input_row(x, &xin[wave][lane * MLP_KIN]);
+    __syncthreads();
+    mlp_forward_wave(xin[wave], wf, bias, bias + 64, bias + 128, bias + 192, yout[wave], nullptr);
+    __syncthreads();
+    const float y0 = yout[wave][lane * 2 + 0], y1 = yout[wave][lane * 2 + 1];
+    if (valid) {
+      double p[8];
+      int iters = 0, which = 0, used = 0;
+      int err = check_cons(u);
+      if (!err) {
+        C2PState s;
+        c2p_new_state(s, u, gamma, pfloor);
+        double Zg, Wg;
+        learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
+        err = c2p_anton2dzw(s, Zg, Wg, p, iters);
+        used = err == 0;
+      }
+      if (err) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) p[q] = first ? 0.0 : P[q * T + z];
+        err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
+      }
+      int failed = err != 0;
+      if (failed && ereset) {
+        p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
+        p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
+        failed = 0;
+      }
+      const int mult = scatter_images(g, P, T, i, j, k, p);
+      cnt[0] += 1;
+      cnt[1] += (unsigned long long)iters;
+      cnt[2] += used ? 0ull : 1ull;
+      cnt[3] += failed ? (unsigned long long)mult : 0ull;
+      cnt[4] += 1;
+      cnt[5] += used ? 0ull : 1ull;
+    }
+  }
+  unsigned long long *const dst[6] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                       &st->stage_failed[pass], &st->mlp_guesses, &st->mlp_fallbacks };
+  block_add<6>(cnt, dst);
 }
 
 /* The learned c2p's network pass over the interior: one wave per 64 zones on the matrix
@@ -901,8 +991,9 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(double2)) != hipSuccess)
     return fail_free(c);
   if (cfg->precision == MARAGPU_MIXED && cfg->riemann == MARAGPU_HLLD) {
-    // room for 1 interface in 256 (FP32 HLLD fails on about 1 in 10^5 in turbulence)
-    const long cap = c->g.T / 256 > 4096 ? c->g.T / 256 : 4096;
+    // room for 1 interface in 64 per sweep: FP32 HLLD fails on about 1 in 10^5 interfaces
+    // at Lorentz factors ~ 1.5 and 1 in 300 at ~ 10 (8 bytes a slot: 0.2 % of a state array)
+    const long cap = c->g.T / 64 > 4096 ? c->g.T / 64 : 4096;
     c->redo.cap = (unsigned int)cap;
     if (hipMalloc(&c->redo.count, sizeof(unsigned int)) != hipSuccess ||
         hipMalloc(&c->redo.z, (size_t)cap * sizeof(long)) != hipSuccess)
@@ -997,10 +1088,17 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
       MlpDev net;
       if (maragpu_mlp_device_params(c->mlp, &net)) return MARAGPU_E_NOMODEL;
       const unsigned nb = nblocks(ni) < 4096u ? nblocks(ni) : 4096u;
-      k_mlp_start<<<nb, kBlock, 0, c->stream>>>(c->U, g, c->cfg.gamma, net, c->start);
-      k_cons_to_prim<true><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
-                                                                  first, c->cfg.ereset, c->cfg.preset,
-                                                                  c->cfg.dreset, c->dstats, pass, c->start);
+      if (kFusedLearnedC2p) {
+        k_cons_to_prim_fused<<<nb, kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor, first,
+                                                           c->cfg.ereset, c->cfg.preset, c->cfg.dreset,
+                                                           c->dstats, pass, net);
+      }
+      else {
+        k_mlp_start<<<nb, kBlock, 0, c->stream>>>(c->U, g, c->cfg.gamma, net, c->start);
+        k_cons_to_prim<true><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+                                                                    first, c->cfg.ereset, c->cfg.preset,
+                                                                    c->cfg.dreset, c->dstats, pass, c->start);
+      }
     }
     else {
       k_cons_to_prim<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
