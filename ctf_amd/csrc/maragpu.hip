@@ -987,7 +987,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (hipMalloc(&c->dstats, sizeof(DevStats)) != hipSuccess) return fail_free(c);
   if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
   if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
-  if (cfg->c2p == MARAGPU_C2P_LEARNED &&
+  if (cfg->c2p == MARAGPU_C2P_LEARNED && !kFusedLearnedC2p &&
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(double2)) != hipSuccess)
     return fail_free(c);
   if (cfg->precision == MARAGPU_MIXED && cfg->riemann == MARAGPU_HLLD) {
