@@ -216,12 +216,19 @@ int mara_set_godunov(const char *key)
   if (int rc = need_app()) return rc;
   if (key && std::strcmp(key, "plm-split") == 0) {
     app()->have_godunov = true;
+    app()->cfg.godunov = MARAGPU_PLM_SPLIT;
     mark_dirty();
     return 0;
   }
-  if (key && (std::strcmp(key, "plm-muscl") == 0 || std::strcmp(key, "weno-split") == 0))
+  if (key && std::strcmp(key, "weno-split") == 0) {
+    app()->have_godunov = true;
+    app()->cfg.godunov = MARAGPU_WENO_SPLIT;
+    mark_dirty();
+    return 0;
+  }
+  if (key && std::strcmp(key, "plm-muscl") == 0)
     return fail(std::string("[Mara] integration scheme '") + key +
-                "' is outside this build (plm-split with extrap pcm|plm|weno5)");
+                "' is outside this build (plm-split, weno-split)");
   return fail(std::string("[Mara] no such integration scheme '") + (key ? key : "(null)") + "'");
 }
 
@@ -247,9 +254,11 @@ int mara_config_solver_str(const char *key, const char *value)
   if (!key || !value) return fail("[Mara] config_solver: null key or value");
   maragpu_config &c = app()->cfg;
   if (std::strcmp(key, "fsplit") == 0) {
-    if (std::strcmp(value, "llf") && std::strcmp(value, "marq"))
-      return fail(std::string("[Mara] no such fsplit '") + value + "'");
-    return 0;  // consumed by weno-split only
+    if (!std::strcmp(value, "llf")) return 0;   // GodunovOperator's default, hydro.cpp:72-73
+    if (!std::strcmp(value, "marq"))
+      // Mara/weno-split.cpp:73-75 hard-codes five fields for the Marquina branch
+      return fail("[Mara] fsplit 'marq' is written for five fields and cannot run rmhd; use 'llf'");
+    return fail(std::string("[Mara] no such fsplit '") + value + "'");
   }
   if (std::strcmp(key, "extrap") == 0) {
     if (!std::strcmp(value, "pcm")) c.recon = MARAGPU_PCM;

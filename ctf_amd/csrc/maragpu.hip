@@ -60,6 +60,7 @@ struct DevStats {
   unsigned long long redo;             // mixed form: interfaces recomputed in FP64
   unsigned long long redo_overflow;    // ... that did not fit the queue (FP32 HLLC kept)
   unsigned long long stage_failed[6];  // failed zones per c2p pass (ghost images included)
+  unsigned long long stage_reset[6];   // zones reset by the emergency repair per pass (weno-split)
 };
 
 /* Mixed form: interfaces whose FP32 HLLD solve failed, recomputed in FP64 by
@@ -208,6 +209,26 @@ __device__ __forceinline__ int scatter_images(const GridDesc &g, double *__restr
   return ni * nj * nk;
 }
 
+/* weno-split reads the conserved state of every zone, ghosts included, as Uglb after
+ * GodunovOperator::ConsToPrim (Mara/weno-split.cpp:26-28): the c2p kernels write U to each
+ * zone's images (Uw). When the emergency repair reset zones in that pass, the reference
+ * rebuilds every zone's U from the repaired primitives (Mara/hydro.cpp:186-193); this
+ * kernel does the same, and returns at once when no zone was reset. */
+__global__ void k_conserved_after_reset(const double *__restrict__ P, double *__restrict__ Uw, GridDesc g,
+                                        double gamma, const unsigned long long *__restrict__ resets)
+{
+  if (*resets == 0ull) return;
+  const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long T = g.T;
+  if (z >= T) return;
+  double p[8], u[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) p[q] = P[q * T + z];
+  prim_to_cons(p, u, gamma - 1.0);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) Uw[q * T + z] = u[q];
+}
+
 /* Per-zone conserved->primitive (GodunovOperator::ConsToPrim, Mara/hydro.cpp:125-202)
  * with the emergency reset of :165-196. `first` = the previous primitive state is the
  * zero-filled PrimitiveArray (first RK stage). LEARNED: each zone first runs anton2dzw
@@ -219,7 +240,8 @@ __global__ __launch_bounds__(kBlock, 5) void k_cons_to_prim(const double *__rest
                                                          double gamma, double pfloor, int first,
                                                          int ereset, double preset, double dreset,
                                                          DevStats *st, int pass,
-                                                         const double2 *__restrict__ start)
+                                                         const double2 *__restrict__ start,
+                                                         double *__restrict__ Uw)
 {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long ni = (long)g.N[0] * g.N[1] * g.N[2];
@@ -258,6 +280,10 @@ __global__ __launch_bounds__(kBlock, 5) void k_cons_to_prim(const double *__rest
       failed = 0;
     }
     const int mult = scatter_images(g, P, T, i, j, k, p);
+    if (Uw) {
+      scatter_images(g, Uw, T, i, j, k, u);
+      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
+    }
     cnt[0] = 1;
     cnt[1] = (unsigned long long)iters;
     cnt[2] = LEARNED ? !used : which != 0;
@@ -304,7 +330,8 @@ __global__ __launch_bounds__(kBlock, 2) void k_cons_to_prim_fused(const double *
                                                                   double *__restrict__ P, GridDesc g,
                                                                   double gamma, double pfloor, int first,
                                                                   int ereset, double preset, double dreset,
-                                                                  DevStats *st, int pass, MlpDev net)
+                                                                  DevStats *st, int pass, MlpDev net,
+                                                                  double *__restrict__ Uw)
 {
   __shared__ __align__(16) uint16_t wf[MLP_NFWD * MLP_FRAG];
   __shared__ __align__(16) float bias[3 * 64 + 2];
@@ -369,6 +396,10 @@ __global__ __launch_bounds__(kBlock, 2) void k_cons_to_prim_fused(const double *
         failed = 0;
       }
       const int mult = scatter_images(g, P, T, i, j, k, p);
+      if (Uw) {
+      scatter_images(g, Uw, T, i, j, k, u);
+      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
+    }
       cnt[0] += 1;
       cnt[1] += (unsigned long long)iters;
       cnt[2] += used ? 0ull : 1ull;
@@ -635,6 +666,83 @@ __global__ __launch_bounds__(kBlock) void k_flux_redo(const double *__restrict__
 }
 
 /* --------------------------------------------------------------------------
+ * weno-split (WenoSplit, Mara/weno-split.cpp) with local Lax-Friedrichs flux splitting.
+ * ------------------------------------------------------------------------ */
+/* Per zone, for the sweep along DIM: the physical flux F and the fastest speed
+ * A = max(|a+|, |a-|) (drive_single_sweep, Mara/weno-split.cpp:224-236), over every zone;
+ * MaxLambda is their maximum. Uw holds U with its ghost images (as Uglb after ConsToPrim). */
+template <typename R, int DIM>
+__global__ __launch_bounds__(kBlock) void k_zone_flux(const double *__restrict__ Uw, const double *__restrict__ P,
+                                                      R *__restrict__ FZ, R *__restrict__ AZ, GridDesc g,
+                                                      PhysParams pp, DevStats *st)
+{
+  const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long T = g.T;
+  double lam_out = 0.0;
+  if (z < T) {
+    R u[8], p[8], f[8], ap, am;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      u[q] = (R)Uw[q * T + z];
+      p[q] = (R)P[q * T + z];
+    }
+    flux_and_eigenvalues<R, DIM>(u, p, f, ap, am, (R)pp.gamma);
+    const R a = (dfabs(ap) > dfabs(am)) ? dfabs(ap) : dfabs(am);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) FZ[q * T + z] = f[q];
+    AZ[z] = a;
+    lam_out = (double)a;
+  }
+  block_max_lambda(lam_out, &st->lam_bits);
+}
+
+/* WenoSplit::intercell_flux_sweep, local Lax-Friedrichs branch (Mara/weno-split.cpp:163-207),
+ * component-wise projection: one thread per interface z | z+S whose index along DIM lies in
+ * [Ng-1, N+Ng-1] (every transverse line, ghosts included). With ml the largest A over the
+ * 6-zone stencil, each field's split fluxes 0.5 (F +- ml U) are reconstructed by the
+ * finite-difference WENO5 (C2R at z of the + part, C2L at z+S of the - part) and summed. The
+ * reference projects with L and R of FluidEquations::Eigensystem; the identity leaves every
+ * value as it is. */
+template <typename R, typename FT, int DIM>
+__global__ __launch_bounds__(kBlock) void k_flux_weno_split(const double *__restrict__ Uw, const R *__restrict__ FZ,
+                                                            const R *__restrict__ AZ, FT *__restrict__ Fout,
+                                                            GridDesc g, PhysParams pp)
+{
+  const long c0 = DIM == 1 ? g.N[0] + 1 : g.n[0];
+  const long c1 = DIM == 2 ? g.N[1] + 1 : g.n[1];
+  const long c2 = DIM == 3 ? g.N[2] + 1 : g.n[2];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c0 * c1 * c2) return;
+  const long k = t % c2 + (DIM == 3 ? g.Ng - 1 : 0);
+  const long j = (t / c2) % c1 + (DIM == 2 ? g.Ng - 1 : 0);
+  const long i = t / (c2 * c1) + (DIM == 1 ? g.Ng - 1 : 0);
+  const long T = g.T;
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  const long z = (i * g.n[1] + j) * g.n[2] + k;
+  R ml = AZ[z - 2 * S];
+#pragma unroll
+  for (int m = -1; m <= 3; ++m) {
+    const R a = AZ[z + m * S];
+    ml = a > ml ? a : ml;
+  }
+  const R h = R(0.5);
+#pragma unroll 1
+  for (int q = 0; q < 8; ++q) {
+    R fp[6], fm[6];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      const long zm = z + (m - 2) * S;
+      const R F = FZ[q * T + zm], U = (R)Uw[q * T + zm];
+      fp[m] = h * (F + ml * U);
+      fm[m] = h * (F - ml * U);
+    }
+    const R f = weno5<R, true, true>(fp[0], fp[1], fp[2], fp[3], fp[4], pp.IS, (R)pp.sz10A) +
+                weno5<R, false, true>(fm[1], fm[2], fm[3], fm[4], fm[5], pp.IS, (R)pp.sz10A);
+    Fout[q * T + z] = (FT)f;
+  }
+}
+
+/* --------------------------------------------------------------------------
  * CT + divergence + RK stage (FP64, no contraction: the same roundings as the reference)
  * ------------------------------------------------------------------------ */
 enum StageKind {
@@ -816,6 +924,8 @@ __global__ void kb_reconstruct(long n, const double *V, double *out, int op, dou
   switch (op) {
   case 0: r = plm<double>(v[-1], v[0], v[1], -1.0, theta); break;  // PLM_C2L
   case 1: r = plm<double>(v[-1], v[0], v[1], +1.0, theta); break;  // PLM_C2R
+  case 2: r = weno5<double, true, true>(v[-2], v[-1], v[0], v[1], v[2], IS, sz10A); break;   // WENO5_FD_C2R
+  case 3: r = weno5<double, false, true>(v[-2], v[-1], v[0], v[1], v[2], IS, sz10A); break;  // WENO5_FD_C2L
   case 4: r = weno5<double, true>(v[-2], v[-1], v[0], v[1], v[2], IS, sz10A); break;
   case 5: r = weno5<double, false>(v[-2], v[-1], v[0], v[1], v[2], IS, sz10A); break;
   default: r = 0.0;
@@ -866,6 +976,10 @@ struct maragpu_ctx {
   double2 *start = nullptr;
   // mixed form with HLLD: the FP64 redo queue of each sweep
   RedoQueue redo = { nullptr, 0u, nullptr };
+  // weno-split: U with its ghost images, and the zone fluxes (8 planes) and speeds (1 plane)
+  // of the current sweep in the flux precision
+  double *Uw = nullptr;
+  void *FZ = nullptr;
 };
 
 namespace {
@@ -935,6 +1049,20 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
   }
 }
 
+/* weno-split sweep along DIM: zone fluxes and speeds, then the split interface fluxes */
+template <typename R, int DIM>
+void launch_weno_split(maragpu_ctx *c, const PhysParams &pp)
+{
+  const GridDesc &g = c->g;
+  R *FZ = static_cast<R *>(c->FZ);
+  R *AZ = FZ + 8 * g.T;
+  k_zone_flux<R, DIM><<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Uw, c->Pw, FZ, AZ, g, pp, c->dstats);
+  const long cnt = (long)(DIM == 1 ? g.N[0] + 1 : g.n[0]) * (DIM == 2 ? g.N[1] + 1 : g.n[1]) *
+                   (DIM == 3 ? g.N[2] + 1 : g.n[2]);
+  k_flux_weno_split<R, R, DIM><<<nblocks(cnt), kBlock, 0, c->stream>>>(c->Uw, FZ, AZ,
+                                                                        static_cast<R *>(c->F[DIM - 1]), g, pp);
+}
+
 struct EventPair { int a, b; };
 
 }  // namespace
@@ -967,6 +1095,9 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (cfg->rk < 1 || cfg->rk > 4 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->recon < 0 ||
       cfg->recon > 2 || cfg->precision < 0 || cfg->precision > 1 || !(cfg->gamma > 1.0))
     return MARAGPU_E_INVALID;
+  if (cfg->godunov != MARAGPU_PLM_SPLIT && cfg->godunov != MARAGPU_WENO_SPLIT) return MARAGPU_E_INVALID;
+  if (cfg->eigen != MARAGPU_EIG_COMPONENT) return MARAGPU_E_INVALID;
+  if (cfg->godunov == MARAGPU_WENO_SPLIT && cfg->Ng < 3) return MARAGPU_E_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MARAGPU_E_HIP;
 
@@ -990,7 +1121,13 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (cfg->c2p == MARAGPU_C2P_LEARNED && !kFusedLearnedC2p &&
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(double2)) != hipSuccess)
     return fail_free(c);
-  if (cfg->precision == MARAGPU_MIXED && cfg->riemann == MARAGPU_HLLD) {
+  if (cfg->godunov == MARAGPU_WENO_SPLIT) {
+    const size_t rb = cfg->precision == MARAGPU_MIXED ? sizeof(float) : sizeof(double);
+    if (hipMalloc(&c->Uw, nd * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->FZ, (size_t)9 * c->g.T * rb) != hipSuccess)
+      return fail_free(c);
+  }
+  if (cfg->precision == MARAGPU_MIXED && cfg->riemann == MARAGPU_HLLD && cfg->godunov != MARAGPU_WENO_SPLIT) {
     // room for 1 interface in 64 per sweep: FP32 HLLD fails on about 1 in 10^5 interfaces
     // at Lorentz factors ~ 1.5 and 1 in 300 at ~ 10 (8 bytes a slot: 0.2 % of a state array)
     const long cap = c->g.T / 64 > 4096 ? c->g.T / 64 : 4096;
@@ -1024,6 +1161,8 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->start);
   hipFree(c->redo.count);
   hipFree(c->redo.z);
+  hipFree(c->Uw);
+  hipFree(c->FZ);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1091,20 +1230,23 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
       if (kFusedLearnedC2p) {
         k_cons_to_prim_fused<<<nb, kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor, first,
                                                            c->cfg.ereset, c->cfg.preset, c->cfg.dreset,
-                                                           c->dstats, pass, net);
+                                                           c->dstats, pass, net, c->Uw);
       }
       else {
         k_mlp_start<<<nb, kBlock, 0, c->stream>>>(c->U, g, c->cfg.gamma, net, c->start);
         k_cons_to_prim<true><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
                                                                     first, c->cfg.ereset, c->cfg.preset,
-                                                                    c->cfg.dreset, c->dstats, pass, c->start);
+                                                                    c->cfg.dreset, c->dstats, pass, c->start, c->Uw);
       }
     }
     else {
       k_cons_to_prim<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
                                                                    first, c->cfg.ereset, c->cfg.preset,
-                                                                   c->cfg.dreset, c->dstats, pass, nullptr);
+                                                                   c->cfg.dreset, c->dstats, pass, nullptr, c->Uw);
     }
+    if (c->Uw && c->cfg.ereset)
+      k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
+                                                                      &c->dstats->stage_reset[pass]);
     HIP_CHECK(hipEventRecord(c->ev[p.b], c->stream));
     return 0;
   };
@@ -1112,7 +1254,19 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     for (int d = 1; d <= 3; ++d) {
       EventPair p = mark(ef);
       hipEventRecord(c->ev[p.a], c->stream);
-      if (mixed) {
+      if (c->Uw) {
+        if (mixed) {
+          if (d == 1) launch_weno_split<float, 1>(c, pp);
+          if (d == 2) launch_weno_split<float, 2>(c, pp);
+          if (d == 3) launch_weno_split<float, 3>(c, pp);
+        }
+        else {
+          if (d == 1) launch_weno_split<double, 1>(c, pp);
+          if (d == 2) launch_weno_split<double, 2>(c, pp);
+          if (d == 3) launch_weno_split<double, 3>(c, pp);
+        }
+      }
+      else if (mixed) {
         if (d == 1) launch_flux<float, float, 1>(c, pp);
         if (d == 2) launch_flux<float, float, 2>(c, pp);
         if (d == 3) launch_flux<float, float, 3>(c, pp);
@@ -1347,7 +1501,7 @@ int maragpu_batch_flux_eigen(size_t n, const double *P, double *F, double *ap, d
 int maragpu_batch_reconstruct(size_t n, const double *V, double *out, int op, double theta, int IS,
                               double sz10A)
 {
-  if (!V || !out || !(op == 0 || op == 1 || op == 4 || op == 5)) return MARAGPU_E_INVALID;
+  if (!V || !out || op < 0 || op > 5) return MARAGPU_E_INVALID;
   if (n == 0) return MARAGPU_OK;
   DevBuf dV, dO;
   if (dalloc(dV, n * 48) || dalloc(dO, n * 8)) return MARAGPU_E_NOMEM;

@@ -606,14 +606,23 @@ template <typename R> __device__ __forceinline__ R plm(R vm, R v0, R vp, R sg, R
 
 /* __weno5, weno.c:97-154, with the FV C2R (right face of v2) or C2L (left face of v2)
  * coefficient sets of weno.c:24-31; vm2..vp2 are v[-2..2]. */
-template <typename R, bool C2R> __device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2,
-                                                                   int IS, R sz10A)
+template <typename R, bool C2R, bool FD = false>
+__device__ __forceinline__ R weno5(R vm2, R vm1, R v0, R vp1, R vp2, int IS, R sz10A)
 {
-  // c[k][j] and d[k]
-  const R c00 = C2R ? R(3./8.) : R(15./8.), c01 = C2R ? R(3./4.) : R(-5./4.), c02 = C2R ? R(-1./8.) : R(3./8.);
-  const R c10 = C2R ? R(-1./8.) : R(3./8.), c11 = C2R ? R(3./4.) : R(3./4.), c12 = C2R ? R(3./8.) : R(-1./8.);
-  const R c20 = C2R ? R(3./8.) : R(-1./8.), c21 = C2R ? R(-5./4.) : R(3./4.), c22 = C2R ? R(15./8.) : R(3./8.);
-  const R d0 = C2R ? R(5./16.) : R(1./16.), d1 = R(5./8.), d2 = C2R ? R(1./16.) : R(5./16.);
+  // c[k][j] and d[k]: finite-volume C2R / C2L, or (FD) the finite-difference flux
+  // coefficients CeesC2R_FD / CeesC2L_FD, DeesC2R_FD / DeesC2L_FD of weno.c:30-38
+  const R c00 = FD ? (C2R ? R(1./3.) : R(11./6.)) : (C2R ? R(3./8.) : R(15./8.));
+  const R c01 = FD ? (C2R ? R(5./6.) : R(-7./6.)) : (C2R ? R(3./4.) : R(-5./4.));
+  const R c02 = FD ? (C2R ? R(-1./6.) : R(1./3.)) : (C2R ? R(-1./8.) : R(3./8.));
+  const R c10 = FD ? (C2R ? R(-1./6.) : R(1./3.)) : (C2R ? R(-1./8.) : R(3./8.));
+  const R c11 = FD ? R(5./6.) : R(3./4.);
+  const R c12 = FD ? (C2R ? R(1./3.) : R(-1./6.)) : (C2R ? R(3./8.) : R(-1./8.));
+  const R c20 = FD ? (C2R ? R(1./3.) : R(-1./6.)) : (C2R ? R(3./8.) : R(-1./8.));
+  const R c21 = FD ? (C2R ? R(-7./6.) : R(5./6.)) : (C2R ? R(-5./4.) : R(3./4.));
+  const R c22 = FD ? (C2R ? R(11./6.) : R(1./3.)) : (C2R ? R(15./8.) : R(3./8.));
+  const R d0 = FD ? (C2R ? R(0.3) : R(0.1)) : (C2R ? R(5./16.) : R(1./16.));
+  const R d1 = FD ? R(0.6) : R(5./8.);
+  const R d2 = FD ? (C2R ? R(0.1) : R(0.3)) : (C2R ? R(1./16.) : R(5./16.));
 
   const R vs0 = c00*v0 + c01*vp1 + c02*vp2;
   const R vs1 = c10*vm1 + c11*v0 + c12*vp1;

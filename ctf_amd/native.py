@@ -25,6 +25,8 @@ IS_MODES = {"js96": JS96, "b08": B08, "sz10": SZ10}
 RIEMANN = {"hll": HLL, "hllc": HLLC, "hlld": HLLD}
 ADVANCE = {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}
 PRECISION = {"fp64": FP64, "mixed": MIXED}
+GODUNOV = {"plm-split": 0, "weno-split": 1}
+EIGEN = {"component": 0, "rmhd": 1}
 C2P = {"plain": C2P_PLAIN, "learned": C2P_LEARNED}
 
 EXPORTS = [
@@ -56,7 +58,8 @@ class Config(C.Structure):
                 ("theta", C.c_double), ("IS", C.c_int), ("sz10A", C.c_double),
                 ("riemann", C.c_int), ("rk", C.c_int), ("precision", C.c_int),
                 ("c2p", C.c_int), ("pfloor", C.c_double), ("ereset", C.c_int),
-                ("preset", C.c_double), ("dreset", C.c_double)]
+                ("preset", C.c_double), ("dreset", C.c_double), ("godunov", C.c_int),
+                ("eigen", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -206,7 +209,7 @@ def check(rc: int, what: str = "maragpu"):
 def make_config(N=(64, 64, 64), Ng=3, x0=(0.0, 0.0, 0.0), x1=(1.0, 1.0, 1.0), gamma=4.0 / 3.0,
                 extrap="plm", theta=2.0, IS="js96", sz10A=50.0, riemann="hlld", advance="rk2",
                 precision="fp64", c2p="plain", pfloor=-1.0, ereset=False, preset=1.0,
-                dreset=1.0) -> Config:
+                dreset=1.0, godunov="plm-split", eigen="component") -> Config:
     c = Config()
     c.N[:] = list(N)
     c.Ng = Ng
@@ -225,6 +228,8 @@ def make_config(N=(64, 64, 64), Ng=3, x0=(0.0, 0.0, 0.0), x1=(1.0, 1.0, 1.0), ga
     c.ereset = int(ereset)
     c.preset = preset
     c.dreset = dreset
+    c.godunov = GODUNOV[godunov]
+    c.eigen = EIGEN[eigen]
     return c
 
 
@@ -316,7 +321,8 @@ def flux_eigen(P, dim, gamma):
 
 
 def reconstruct(V, op, theta=2.0, IS="js96", sz10A=50.0):
-    ops = {"PLM_C2L": 0, "PLM_C2R": 1, "WENO5_FV_C2R": 4, "WENO5_FV_C2L": 5}
+    ops = {"PLM_C2L": 0, "PLM_C2R": 1, "WENO5_FD_C2R": 2, "WENO5_FD_C2L": 3, "WENO5_FV_C2R": 4,
+           "WENO5_FV_C2L": 5}
     V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 6)
     out = np.zeros(len(V))
     check(lib().maragpu_batch_reconstruct(len(V), dptr(V), dptr(out), ops[op], theta,

@@ -14,7 +14,10 @@
  *                                      host layout = Mara's AoS [i][j][k][q] (Mara/valman.hpp:25-36)
  *   maragpu_advance                    luaC_Mara_advance body (Mara/mara.cpp:297-341):
  *                                      GodunovOperator::PrimToCons, RungeKuttaIntegration::AdvanceState,
- *                                      ConsToPrim; state is not committed when zones fail
+ *                                      ConsToPrim; state is not committed when zones fail. With
+ *                                      godunov = MARAGPU_WENO_SPLIT the operator is WenoSplit with
+ *                                      local Lax-Friedrichs splitting (set_godunov('weno-split'),
+ *                                      fsplit = 'llf'; Mara/weno-split.cpp:13-245)
  *   maragpu_get_timestep               luaC_Mara_get_timestep (Mara/mara.cpp:374-381)
  *   maragpu_measure_divB               luaC_mean_max_divB (Mara/mara.cpp:2264-2324)
  *   maragpu_batch_prim_to_cons         AdiabaticIdealRmhd::PrimToCons (Mara/rmhd.cpp:133-163)
@@ -40,6 +43,8 @@ enum { MARAGPU_JS96 = 0, MARAGPU_B08 = 1, MARAGPU_SZ10 = 2 };            /* conf
 enum { MARAGPU_HLL = 0, MARAGPU_HLLC = 1, MARAGPU_HLLD = 2 };            /* set_riemann */
 enum { MARAGPU_FP64 = 0, MARAGPU_MIXED = 1 };                            /* precision */
 enum { MARAGPU_C2P_PLAIN = 0, MARAGPU_C2P_LEARNED = 1 };                 /* c2p initial guess */
+enum { MARAGPU_PLM_SPLIT = 0, MARAGPU_WENO_SPLIT = 1 };                  /* set_godunov */
+enum { MARAGPU_EIG_COMPONENT = 0, MARAGPU_EIG_RMHD = 1 };                /* weno-split projection */
 
 enum {
   MARAGPU_OK = 0,
@@ -66,6 +71,8 @@ typedef struct maragpu_config {
   double pfloor;     /* pressure floor, disabled if <= 0 */
   int ereset;        /* emergency reset of failed zones */
   double preset, dreset;
+  int godunov;       /* MARAGPU_PLM_SPLIT or MARAGPU_WENO_SPLIT (local Lax-Friedrichs splitting) */
+  int eigen;         /* weno-split projection: MARAGPU_EIG_COMPONENT */
 } maragpu_config;
 
 typedef struct maragpu_stats {
@@ -112,7 +119,7 @@ int maragpu_batch_cons_to_prim(size_t n, const double *U, double *P, double gamm
 int maragpu_batch_flux_eigen(size_t n, const double *P, double *F, double *ap, double *am, int dim,
                              double gamma);
 /* V holds n stencils of 6 values v[-2..3]; op as weno.h ReconstructOperation (PLM_C2L=0,
- * PLM_C2R=1, WENO5_FV_C2R=4, WENO5_FV_C2L=5) applied at v[0] */
+ * PLM_C2R=1, WENO5_FD_C2R=2, WENO5_FD_C2L=3, WENO5_FV_C2R=4, WENO5_FV_C2L=5) applied at v[0] */
 int maragpu_batch_reconstruct(size_t n, const double *V, double *out, int op, double theta, int IS,
                               double sz10A);
 int maragpu_batch_riemann(size_t n, int solver, int precision, const double *PL, const double *PR,

@@ -51,6 +51,8 @@ typedef struct {
   double pfloor;
   int ereset;
   double preset, dreset;
+  int godunov;    /* 0 plm-split, 1 weno-split (local Lax-Friedrichs flux splitting) */
+  int eigen;      /* weno-split projection: 0 component-wise, 1 RMHD characteristic */
 } oracle_config;
 
 typedef struct {
@@ -175,9 +177,11 @@ int cons_to_prim_all(Grid &g, double *U, double *P, Acc &acc)
         p[5] = U[8 * z + 5];
         p[6] = U[8 * z + 6];
         p[7] = U[8 * z + 7];
-        prim_to_cons(p, U + 8 * z, g.pp.gamma);
       }
     }
+    // every zone's U is rebuilt from the repaired primitives (:186-193)
+#pragma omp parallel for schedule(static)
+    for (long z = 0; z < nz; ++z) prim_to_cons(P + 8 * z, U + 8 * z, g.pp.gamma);
     std::fill(g.failure_mask.begin(), g.failure_mask.end(), 0);
     ttl = 0;
   }
@@ -235,6 +239,92 @@ void flux_sweep(Grid &g, const double *P, double *F, int dim, Acc &acc)
   acc.failures += failures;
 }
 
+/* The projection of the weno-split sweep: the identity (component-wise splitting), or
+ * (eigen = 1) the RMHD characteristic eigenvectors. The reference's RMHD fluid has
+ * no Eigensystem (FluidEquations::Eigensystem, Mara/hydro.hpp:146-147, is empty), so its
+ * weno-split reads uninitialised matrices; these are the two working choices. */
+void eigensystem(const Grid &g, const double *U, const double *P, double *L, double *R, double *lam,
+                 int dim)
+{
+  (void)U; (void)P; (void)dim;
+  for (int i = 0; i < 8; ++i) {
+    lam[i] = 0.0;
+    for (int j = 0; j < 8; ++j) L[i*8 + j] = R[i*8 + j] = (i == j) ? 1.0 : 0.0;
+  }
+}
+
+/* matrix_vector_product, Mara/matmult.c:41-62: b_i = sum_j A_ij x_j, summed in j order */
+inline void matvec8(const double *A, const double *x, double *b)
+{
+  for (int i = 0; i < 8; ++i) {
+    b[i] = 0.0;
+    for (int j = 0; j < 8; ++j) b[i] += A[i*8 + j] * x[j];
+  }
+}
+
+/* WenoSplit::drive_single_sweep and intercell_flux_sweep with local Lax-Friedrichs flux
+ * splitting, Mara/weno-split.cpp:163-207 and :210-245, over every line of the sweep
+ * direction (ghost lines included, :296-314). Per line: the physical flux F and the
+ * fastest speed A of every zone (MaxLambda); per interface i in [Ng-1, N+Ng-1]: the
+ * eigensystem of the averaged primitive state, the split fluxes 0.5 (F +- ml U) of the
+ * 6-zone stencil (ml = max A over it) projected with L, the finite-difference WENO5 of
+ * each projected component and the projection back with R. The reference leaves the
+ * other interfaces of a line unwritten (malloc'd); they only feed ghost zones, which the
+ * next ConsToPrim overwrites; zero is used here. */
+void weno_split_sweep(Grid &g, const double *U, const double *P, double *Fiph, int dim, Acc &acc)
+{
+  const int Ng = g.cfg.Ng;
+  const int Nd = g.cfg.N[dim - 1];
+  const int n = Nd + 2 * Ng;
+  const long S = dim == 1 ? g.sx : dim == 2 ? g.sy : g.sz;
+  // the two transverse axes and their strides (in doubles)
+  const int a1 = dim == 1 ? 2 : 1, a2 = dim == 3 ? 2 : 3;
+  const long s1 = a1 == 1 ? g.sx : g.sy, s2 = a2 == 2 ? g.sy : g.sz;
+  const int n1 = g.n[a1 - 1], n2 = g.n[a2 - 1];
+  double lam_max = 0.0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(max : lam_max)
+  for (long line = 0; line < (long)n1 * n2; ++line) {
+    const long base = (line / n2) * s1 + (line % n2) * s2;
+    std::vector<double> F(8 * n), A(n);
+    for (int i = 0; i < n; ++i) {
+      double ap, am;
+      flux_and_eigenvalues(U + base + i * S, P + base + i * S, &F[8 * i], &ap, &am, dim, g.pp.gamma);
+      A[i] = (std::fabs(ap) > std::fabs(am)) ? std::fabs(ap) : std::fabs(am);
+      if (lam_max < A[i]) lam_max = A[i];
+    }
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < 8; ++q) Fiph[base + i * S + q] = 0.0;
+    for (int i = Ng - 1; i < Nd + Ng; ++i) {
+      double Piph[8], Uiph[8], L[64], R[64], lam[8];
+      for (int q = 0; q < 8; ++q) Piph[q] = 0.5 * (P[base + i * S + q] + P[base + (i + 1) * S + q]);
+      prim_to_cons(Piph, Uiph, g.pp.gamma);
+      eigensystem(g, Uiph, Piph, L, R, lam, dim);
+      const double ml = *std::max_element(&A[i - 2], &A[i + 4]);
+      double fpT[8 * 6], fmT[8 * 6];
+      for (int j = 0; j < 6; ++j) {
+        const double *Uj = U + base + (i + j - 2) * S;
+        const double *Fj = &F[8 * (i + j - 2)];
+        double Fp[8], Fm[8], fp[8], fm[8];
+        for (int q = 0; q < 8; ++q) {
+          Fp[q] = 0.5 * (Fj[q] + ml * Uj[q]);
+          Fm[q] = 0.5 * (Fj[q] - ml * Uj[q]);
+        }
+        matvec8(L, Fp, fp);
+        matvec8(L, Fm, fm);
+        for (int q = 0; q < 8; ++q) {
+          fpT[q * 6 + j] = fp[q];
+          fmT[q * 6 + j] = fm[q];
+        }
+      }
+      double f[8];
+      for (int q = 0; q < 8; ++q)
+        f[q] = reconstruct(&fpT[q * 6 + 2], WENO5_FD_C2R, g.pp) + reconstruct(&fmT[q * 6 + 3], WENO5_FD_C2L, g.pp);
+      matvec8(R, f, &Fiph[base + i * S]);
+    }
+  }
+  if (lam_max > acc.max_lambda) acc.max_lambda = lam_max;
+}
+
 /* Rmhd::ConstrainedTransport3d, Mara/rmhd.cpp:293-337. The reference's temporaries are
  * uninitialised outside [sx, stride0-sx); those zones are ghosts, zero is used here. */
 void constrained_transport_3d(const Grid &g, double *Fx, double *Fy, double *Fz)
@@ -278,9 +368,17 @@ bool dUdt(Grid &g, const std::vector<double> &Uin, std::vector<double> &L, Acc &
   if (e != 0) { *err = e; return false; }
   const long s0 = 8 * g.T;
   std::vector<double> F(s0, 0.0), G(s0, 0.0), H(s0, 0.0);
-  flux_sweep(g, P.data(), F.data(), 1, acc);
-  flux_sweep(g, P.data(), G.data(), 2, acc);
-  flux_sweep(g, P.data(), H.data(), 3, acc);
+  if (g.cfg.godunov == 1) {
+    // WenoSplit::drive_sweeps_3d, Mara/weno-split.cpp:291-335 (U is the synchronised copy)
+    weno_split_sweep(g, U.data(), P.data(), F.data(), 1, acc);
+    weno_split_sweep(g, U.data(), P.data(), G.data(), 2, acc);
+    weno_split_sweep(g, U.data(), P.data(), H.data(), 3, acc);
+  }
+  else {
+    flux_sweep(g, P.data(), F.data(), 1, acc);
+    flux_sweep(g, P.data(), G.data(), 2, acc);
+    flux_sweep(g, P.data(), H.data(), 3, acc);
+  }
   constrained_transport_3d(g, F.data(), G.data(), H.data());
   const long sx = g.sx, sy = g.sy, sz = g.sz;
   const double dx = g.dx, dy = g.dy, dz = g.dz;

@@ -26,6 +26,8 @@ REF_SO = os.path.join(HERE, "_ref", "libmararef.so")
 RECON = {"pcm": 0, "plm": 1, "weno5": 2}
 IS_MODES = {"js96": 0, "b08": 1, "sz10": 2}
 RIEMANN = {"hll": 0, "hllc": 1, "hlld": 2}
+GODUNOV = {"plm-split": 0, "weno-split": 1}
+EIGEN = {"component": 0, "rmhd": 1}
 ADVANCE = {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}
 # weno.h ReconstructOperation
 OPS = {"PLM_C2L": 0, "PLM_C2R": 1, "WENO5_FD_C2R": 2, "WENO5_FD_C2L": 3,
@@ -37,7 +39,8 @@ class Config(C.Structure):
                 ("x1", C.c_double * 3), ("gamma", C.c_double), ("recon", C.c_int),
                 ("theta", C.c_double), ("IS", C.c_int), ("sz10A", C.c_double),
                 ("riemann", C.c_int), ("rk", C.c_int), ("pfloor", C.c_double),
-                ("ereset", C.c_int), ("preset", C.c_double), ("dreset", C.c_double)]
+                ("ereset", C.c_int), ("preset", C.c_double), ("dreset", C.c_double),
+                ("godunov", C.c_int), ("eigen", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -65,6 +68,8 @@ class Setup:
     ereset: bool = False
     preset: float = 1.0
     dreset: float = 1.0
+    godunov: str = "plm-split"
+    eigen: str = "component"
 
     def to_c(self) -> Config:
         c = Config()
@@ -83,6 +88,8 @@ class Setup:
         c.ereset = int(self.ereset)
         c.preset = self.preset
         c.dreset = self.dreset
+        c.godunov = GODUNOV[self.godunov]
+        c.eigen = EIGEN[self.eigen]
         return c
 
 

@@ -28,6 +28,8 @@ typedef struct {
   double pfloor;
   int ereset;
   double preset, dreset;
+  int godunov;    /* 0 plm-split, 1 weno-split */
+  int eigen;      /* weno-split projection: 0 component-wise, 1 RMHD characteristic */
 } oracle_config;
 
 typedef struct {
@@ -45,13 +47,34 @@ typedef struct {
 
 static MaraApplication *App = nullptr;
 
-static void ref_start(double gamma)
+/* The reference's RMHD fluid with an Eigensystem: FluidEquations::Eigensystem
+ * (Mara/hydro.hpp:146-147) is empty for it, so WenoSplit would read uninitialised
+ * matrices. Mode 0 supplies the identity (component-wise splitting), which lets the
+ * reference's own weno-split sweep run on RMHD. */
+class RmhdWithEigensystem : public AdiabaticIdealRmhd {
+public:
+  int mode = 0;
+  void Eigensystem(const double *U, const double *P, double *L, double *R, double *lam,
+                   int dim) const override
+  {
+    (void)U; (void)P; (void)dim;
+    for (int i = 0; i < 8; ++i) {
+      lam[i] = 0.0;
+      for (int j = 0; j < 8; ++j) L[i*8 + j] = R[i*8 + j] = (i == j) ? 1.0 : 0.0;
+    }
+  }
+};
+
+static void ref_start(double gamma, bool with_eigensystem = false)
 {
   if (App) delete App;
   App = new MaraApplication;
   HydroModule::Mara = App;
   delete App->fluid;
-  App->fluid = new AdiabaticIdealRmhd;
+  // HllcRiemannSolver dispatches on typeid(fluid) == typeid(AdiabaticIdealRmhd)
+  // (Mara/riemann_hllc.cpp:11-21), so the subclass is used only where no Riemann solver runs
+  if (with_eigensystem) App->fluid = new RmhdWithEigensystem;
+  else App->fluid = new AdiabaticIdealRmhd;
   delete App->eos;
   App->eos = new AdiabaticEos(gamma);
   rmhd_c2p_set_gamma(gamma);
@@ -77,8 +100,13 @@ extern "C" {
 
 void *ref_create(const oracle_config *c)
 {
-  ref_start(c->gamma);
-  App->godunov = new MethodOfLinesSplit;
+  ref_start(c->gamma, c->godunov == 1);
+  if (c->godunov == 1) {
+    App->godunov = new WenoSplit;
+    GodunovOperator::fluxsplit_method = GodunovOperator::FLUXSPLIT_LOCAL_LAX_FRIEDRICHS;
+    static_cast<RmhdWithEigensystem *>(App->fluid)->mode = c->eigen;
+  }
+  else App->godunov = new MethodOfLinesSplit;
   App->boundary = new PeriodicBoundary;
   App->riemann = make_riemann(c->riemann);
   switch (c->rk) {

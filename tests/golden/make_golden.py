@@ -112,12 +112,33 @@ GRID_CASES = [
      dict(extrap="plm", riemann="hlld", advance="rk2", ereset=True, preset=0.5, dreset=2.0), 2),
     ("plm_hlld_rk2_cold_pfloor", "cold", dict(extrap="plm", riemann="hlld", advance="rk2",
                                              pfloor=1e-3), 2),
+    # weno-split (local Lax-Friedrichs splitting), run by the reference's own WenoSplit with the
+    # identity eigensystem supplied by oracle/ref_harness.cpp (component-wise projection)
+    ("weno_split_rk3", "smooth", dict(godunov="weno-split", advance="rk3"), 3),
+    ("weno_split_rk2_jump", "jump", dict(godunov="weno-split", advance="rk2"), 3),
+    ("weno_split_sz10_rk3", "smooth", dict(godunov="weno-split", IS="sz10", sz10A=100.0,
+                                           advance="rk3"), 2),
+    ("weno_split_rk2_superluminal_ereset", "superluminal",
+     dict(godunov="weno-split", advance="rk2", ereset=True, preset=0.5, dreset=2.0), 2),
 ]
 
 
-def main():
+def save(write, path, **arrays):
+    if write:
+        np.savez_compressed(path, **arrays)
+
+
+def main(only=None):
+    """Writes every fixture, or with `only` (grid-case names) just those grid cases; the
+    random stream, hence each case's initial data, is the same either way."""
     ref = Lib("ref")
     rng = np.random.default_rng(20240601)
+    write_kats(ref, rng, write=not only)
+    write_grids(ref, rng, only)
+    print("golden vectors written to", HERE)
+
+
+def write_kats(ref, rng, write=True):
     out = {}
 
     # --- conserved -> primitive -------------------------------------------------
@@ -129,7 +150,7 @@ def main():
     Pn, en, inn = ref.c2p_solver(U, 1, GAMMA)
     Pz, ez, iz = ref.cons_to_prim(U, GAMMA, guess=np.zeros_like(P))
     Pf, ef, itf = ref.cons_to_prim(U, GAMMA, pfloor=0.05, guess=np.zeros_like(P))
-    np.savez_compressed(os.path.join(HERE, "kat_c2p.npz"), P=P, U=U, p2c_err=perr, guess=guess,
+    save(write, os.path.join(HERE, "kat_c2p.npz"), P=P, U=U, p2c_err=perr, guess=guess,
                         cascade_P=Pc, cascade_err=ec, anton_P=Pa, anton_err=ea, anton_iters=ia,
                         noble_P=Pn, noble_err=en, noble_iters=inn, zero_guess_P=Pz,
                         zero_guess_err=ez, pfloor=0.05, pfloor_P=Pf, pfloor_err=ef, gamma=GAMMA)
@@ -140,7 +161,7 @@ def main():
     for dim in (1, 2, 3):
         F, ap, am = ref.flux_eigen(P, dim, GAMMA)
         d[f"F{dim}"], d[f"ap{dim}"], d[f"am{dim}"] = F, ap, am
-    np.savez_compressed(os.path.join(HERE, "kat_flux.npz"), **d)
+    save(write, os.path.join(HERE, "kat_flux.npz"), **d)
 
     # --- quartic ------------------------------------------------------------------
     C = rng.normal(size=(300, 5))
@@ -151,7 +172,7 @@ def main():
         r = ref.quartic(*c)
         nr[i] = len(r)
         roots[i, :len(r)] = r
-    np.savez_compressed(os.path.join(HERE, "kat_quartic.npz"), coeffs=C, roots=roots, nroots=nr)
+    save(write, os.path.join(HERE, "kat_quartic.npz"), coeffs=C, roots=roots, nroots=nr)
 
     # --- reconstruction -------------------------------------------------------------
     V = np.concatenate([rng.normal(size=(300, 6)),
@@ -161,10 +182,10 @@ def main():
     for op in ("PLM_C2L", "PLM_C2R"):
         for th in (1.0, 1.5, 2.0):
             d[f"{op}_theta{th}"] = ref.reconstruct(V, op, theta=th)
-    for op in ("WENO5_FV_C2R", "WENO5_FV_C2L"):
+    for op in ("WENO5_FV_C2R", "WENO5_FV_C2L", "WENO5_FD_C2R", "WENO5_FD_C2L"):
         for IS in ("js96", "b08", "sz10"):
             d[f"{op}_{IS}"] = ref.reconstruct(V, op, IS=IS, sz10A=100.0)
-    np.savez_compressed(os.path.join(HERE, "kat_reconstruct.npz"), **d)
+    save(write, os.path.join(HERE, "kat_reconstruct.npz"), **d)
 
     # --- Riemann solvers --------------------------------------------------------------
     PL, PR = riemann_pairs(rng, 400)
@@ -175,14 +196,19 @@ def main():
             d[f"{s}_F{dim}"], d[f"{s}_err{dim}"], d[f"{s}_lam{dim}"] = F, err, lam
             if s == "hlld":
                 d[f"hlld_ok{dim}"] = ok
-    np.savez_compressed(os.path.join(HERE, "kat_riemann.npz"), **d)
+    save(write, os.path.join(HERE, "kat_riemann.npz"), **d)
 
+
+
+def write_grids(ref, rng, only):
     # --- full steps on small grids ------------------------------------------------------
     N = 8
     for name, kind, kw, steps in GRID_CASES:
         st = Setup(N=(N, N, N), **kw)
-        solver = ref.solver(st)
         P0 = grid_state(N, kind, rng)
+        if only and name not in only:
+            continue
+        solver = ref.solver(st)
         P = P0.copy()
         dts, errs, lams, succ, fail = [], [], [], [], []
         dt = 0.0
@@ -202,8 +228,8 @@ def main():
                             max_lambda=np.array(lams), hlld_success=np.array(succ),
                             hlld_failure=np.array(fail), next_dt=dt,
                             setup=np.array(repr(setup)))
-    print("golden vectors written to", HERE)
 
 
 if __name__ == "__main__":
-    main()
+    import sys
+    main(sys.argv[1].split(",") if len(sys.argv) > 1 else None)

@@ -99,7 +99,7 @@ def test_reconstruct(gpu):
             out = gpu.reconstruct(g["V"], op, theta=th)
             ref = g[f"{op}_theta{th}"]
             assert np.allclose(out, ref, rtol=1e-13, atol=1e-14)
-    for op in ("WENO5_FV_C2R", "WENO5_FV_C2L"):
+    for op in ("WENO5_FV_C2R", "WENO5_FV_C2L", "WENO5_FD_C2R", "WENO5_FD_C2L"):
         for IS in ("js96", "b08", "sz10"):
             out = gpu.reconstruct(g["V"], op, IS=IS, sz10A=100.0)
             assert np.allclose(out, g[f"{op}_{IS}"], rtol=1e-11, atol=1e-13)
@@ -143,7 +143,7 @@ def setup_to_config(gpu, s: Setup, **extra):
     return gpu.make_config(N=s.N, Ng=s.Ng, x0=s.x0, x1=s.x1, gamma=s.gamma, extrap=s.extrap,
                            theta=s.theta, IS=s.IS, sz10A=s.sz10A, riemann=s.riemann,
                            advance=s.advance, pfloor=s.pfloor, ereset=s.ereset, preset=s.preset,
-                           dreset=s.dreset, **extra)
+                           dreset=s.dreset, godunov=s.godunov, eigen=s.eigen, **extra)
 
 
 def compare_states(Pg, Po, l1=1e-10, mx=1e-7):
@@ -162,7 +162,8 @@ def compare_states(Pg, Po, l1=1e-10, mx=1e-7):
 GRIDS = ["plm_hlld_rk2", "weno5_hlld_rk3", "plm_hllc_rk2", "pcm_hll_single", "plm_hlld_rk4",
          "weno5_b08_hlld_rk3", "weno5_sz10_hlld_rk3", "plm_hlld_rk2_jump", "plm_theta1_hlld_rk2",
          "plm_hlld_rk2_superluminal", "plm_hlld_rk2_superluminal_ereset",
-         "plm_hlld_rk2_cold_pfloor"]
+         "plm_hlld_rk2_cold_pfloor", "weno_split_rk3", "weno_split_rk2_jump", "weno_split_sz10_rk3",
+         "weno_split_rk2_superluminal_ereset"]
 
 
 @pytest.mark.parametrize("name", GRIDS)
@@ -219,6 +220,37 @@ def test_turbulence_steps_vs_oracle(gpu, oracle_lib, extrap, riemann, advance, N
         dt = so.get_timestep(0.4)
         assert abs(ctx.get_timestep(0.4) - dt) <= 1e-12 * dt
     compare_states(ctx.download(), Po)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_weno_split_steps_vs_oracle(gpu, oracle_lib, precision):
+    """weno-split (local Lax-Friedrichs, component-wise) on seeded turbulence against the
+    oracle, which is pinned bitwise to the reference's own WenoSplit. FP64: the plain-form
+    tolerances; mixed (FP32 zone fluxes and WENO, FP64 c2p/RK/CT): L1 <= 1e-6."""
+    N = 20
+    P0 = gpu.init_turbulence((N, N, N), v_rms=0.4, v_kmax=4.0, four_velocity=True, dB_rms=0.3,
+                             A_kmax=3.0, seed=7)
+    s = Setup(N=(N, N, N), godunov="weno-split", advance="rk3")
+    so = oracle_lib.solver(s)
+    ctx = gpu.Context(setup_to_config(gpu, s, precision=precision))
+    ctx.upload(P0)
+    Po = P0.copy()
+    dt = 0.0
+    for _ in range(4):
+        eo, sto = so.advance(Po, dt)
+        eg, stg = ctx.advance(dt)
+        assert eo == eg == 0
+        tol = 1e-12 if precision == "fp64" else 1e-5
+        assert abs(stg.max_lambda - sto.max_lambda) <= tol * sto.max_lambda
+        dt = so.get_timestep(0.4)
+    if precision == "fp64":
+        compare_states(ctx.download(), Po)
+    else:
+        it = (slice(3, -3),) * 3
+        a, b = ctx.download()[it], Po[it]
+        L1 = np.abs(a - b).sum(axis=(0, 1, 2)) / np.abs(b).sum(axis=(0, 1, 2))
+        assert L1.max() <= 1e-6, L1
+        assert abs(ctx.divB()[2]) < 1e-11
 
 
 def test_divB_measure(gpu, oracle_lib):

@@ -97,6 +97,12 @@ def test_lua_verbs_reject_like_mara():
         Mara.config_solver({"extrap": "ppm"}, True)
     with pytest.raises(MaraError, match="no such IS"):
         Mara.config_solver({"IS": "xyz"}, True)
+    with pytest.raises(MaraError, match="outside this build"):
+        Mara.set_godunov("plm-muscl")
+    with pytest.raises(MaraError, match="five fields"):
+        Mara.config_solver({"fsplit": "marq"}, True)
+    Mara.set_godunov("weno-split")
+    Mara.config_solver({"fsplit": "llf"}, True)
     Mara.config_solver({"unknown_key": 3.0, "precision": "mixed", "c2p": "plain"}, True)
     with pytest.raises(MaraError, match="need a domain"):
         Mara.advance(np.zeros(8), 0.0)
