@@ -246,7 +246,8 @@ void flux_sweep(Grid &g, const double *P, double *F, int dim, Acc &acc)
 void eigensystem(const Grid &g, const double *U, const double *P, double *L, double *R, double *lam,
                  int dim)
 {
-  (void)U; (void)P; (void)dim;
+  if (g.cfg.eigen == 1 && rmhd_eigensystem(U, P, L, R, lam, dim, g.pp.gamma) == 0) return;
+  // component-wise, or where the RMHD eigenvectors are too ill-conditioned to use
   for (int i = 0; i < 8; ++i) {
     lam[i] = 0.0;
     for (int j = 0; j < 8; ++j) L[i*8 + j] = R[i*8 + j] = (i == j) ? 1.0 : 0.0;
@@ -605,6 +606,19 @@ void oracle_c2p_from_start_batch(long n, const double *U, const double *start, d
     int e = s.solve_anton2dzw(P + 8 * z);
     if (err) err[z] = e;
     if (iters) iters[z] = s.iterations;
+  }
+}
+
+/* RMHD characteristic eigensystem of n primitive states (L, R: 64 doubles each, row-major;
+ * lam: 8; ok: 1 where usable) */
+void oracle_eigensystem_batch(long n, const double *P, double *L, double *R, double *lam, int *ok,
+                              int dim, double gamma)
+{
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long z = 0; z < n; ++z) {
+    double U[8];
+    prim_to_cons(P + 8 * z, U, gamma);
+    ok[z] = rmhd_eigensystem(U, P + 8 * z, L + 64 * z, R + 64 * z, lam + 8 * z, dim, gamma) == 0;
   }
 }
 

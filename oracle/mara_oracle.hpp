@@ -15,6 +15,7 @@
 #ifndef ORACLE_MARA_ORACLE_HPP
 #define ORACLE_MARA_ORACLE_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <cstdint>
@@ -1094,3 +1095,236 @@ template <class R> inline int riemann_flux(int solver, const R *pl, const R *pr,
 } // namespace oracle
 
 #endif
+
+namespace oracle {
+
+/* ---------------------------------------------------------------------------------
+ * RMHD characteristic eigensystem for the weno-split projection. No reference
+ * counterpart: FluidEquations::Eigensystem (Mara/hydro.hpp:146-147) is empty for the RMHD
+ * fluid. Pinned by invariants only (tests/test_eigensystem.py): A R = R diag(lam) for the
+ * centred-difference Jacobian A = dF/dU, L R = I, the eigenvalues are the entropy speed,
+ * the two Alfven speeds, the four roots of FluxAndEigenvalues' quartic and zero (the
+ * normal field), and for a uniform state the projected split fluxes reproduce the
+ * component-wise ones.
+ *
+ * Method: M = dU/dP and B = dF/dP by centred differences; A_P = M^-1 B (similar to dF/dU);
+ * eigenvalues in closed form where they have one and otherwise by Newton on the factored
+ * quartic plus deflation; the eigenvectors of each eigenvalue cluster span the null space
+ * of A_P - lam I (Gaussian elimination with complete pivoting); R = M R_P with unit
+ * max-norm columns and L = R^-1. Returns 0, or nonzero when the eigenvectors are too
+ * ill-conditioned to use (the caller falls back to the identity).
+ * ------------------------------------------------------------------------------- */
+namespace eig {
+
+inline void state_fluxes(const double *P, double *U, double *F, int dim, double gamma)
+{
+  prim_to_cons(P, U, gamma);
+  flux_and_eigenvalues<double>(U, P, F, nullptr, nullptr, dim, gamma);
+}
+
+/* solves A X = Bm in place (A n x n, Bm n x m, row-major) by partial pivoting; 0 on success */
+inline int solve(double *A, double *Bm, int n, int m)
+{
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r) if (std::fabs(A[r*n + c]) > std::fabs(A[piv*n + c])) piv = r;
+    if (!(std::fabs(A[piv*n + c]) > 0.0)) return 1;
+    if (piv != c) {
+      for (int k = 0; k < n; ++k) std::swap(A[c*n + k], A[piv*n + k]);
+      for (int k = 0; k < m; ++k) std::swap(Bm[c*m + k], Bm[piv*m + k]);
+    }
+    const double id = 1.0 / A[c*n + c];
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = A[r*n + c] * id;
+      if (f == 0.0) continue;
+      for (int k = c; k < n; ++k) A[r*n + k] -= f * A[c*n + k];
+      for (int k = 0; k < m; ++k) Bm[r*m + k] -= f * Bm[c*m + k];
+    }
+  }
+  for (int r = 0; r < n; ++r) {
+    const double id = 1.0 / A[r*n + r];
+    for (int k = 0; k < m; ++k) Bm[r*m + k] *= id;
+  }
+  return 0;
+}
+
+/* the `want` null vectors of the 8x8 matrix A (destroyed), whose null space has that
+ * dimension: 8 - want steps of Gaussian elimination with complete pivoting, then back
+ * substitution with each remaining column set to one in turn; written to the columns
+ * col0.. of V (8x8, row-major). Returns 1 if a pivot vanishes. */
+inline int null_space(double *A, double *V, int col0, int want)
+{
+  int rowp[8], colp[8];
+  for (int i = 0; i < 8; ++i) { rowp[i] = i; colp[i] = i; }
+  const int rank = 8 - want;
+  for (int k = 0; k < rank; ++k) {
+    int pr = k, pc = k;
+    double best = -1.0;
+    for (int r = k; r < 8; ++r)
+      for (int c = k; c < 8; ++c) {
+        const double a = std::fabs(A[rowp[r]*8 + colp[c]]);
+        if (a > best) { best = a; pr = r; pc = c; }
+      }
+    if (!(best > 0.0)) return 1;
+    std::swap(rowp[k], rowp[pr]);
+    std::swap(colp[k], colp[pc]);
+    const double piv = A[rowp[k]*8 + colp[k]];
+    for (int r = k + 1; r < 8; ++r) {
+      const double f = A[rowp[r]*8 + colp[k]] / piv;
+      for (int c = k; c < 8; ++c) A[rowp[r]*8 + colp[c]] -= f * A[rowp[k]*8 + colp[c]];
+    }
+  }
+  for (int fcount = 0; fcount < want; ++fcount) {
+    double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    x[colp[rank + fcount]] = 1.0;
+    for (int k = rank - 1; k >= 0; --k) {
+      double s = 0.0;
+      for (int c = k + 1; c < 8; ++c) s += A[rowp[k]*8 + colp[c]] * x[colp[c]];
+      x[colp[k]] = -s / A[rowp[k]*8 + colp[k]];
+    }
+    for (int i = 0; i < 8; ++i) V[i*8 + col0 + fcount] = x[i];
+  }
+  return 0;
+}
+
+/* root of K (x-v)^4 + (1-x^2)[L (x-v)^2 + (b0 x - bi)^2] by Newton from x (from outside the
+ * roots it moves monotonically onto the extreme one; from a deflation estimate it polishes) */
+inline double quartic_root_outside(double K, double L, double b0, double bi, double v, double x)
+{
+  for (int it = 0; it < 60; ++it) {
+    const double y = x - v, y2 = y*y, c = b0*x - bi, q1 = L*y2 + c*c, om = (1.0 - x)*(1.0 + x);
+    const double Q = K*y2*y2 + om*q1;
+    const double dQ = 4.0*K*y2*y - 2.0*x*q1 + om*(2.0*L*y + 2.0*b0*c);
+    const double step = Q / dQ;
+    const double xn = x - step;
+    if (!(std::fabs(xn) <= 1.0)) break;
+    x = xn;
+    if (!(std::fabs(step) > 1e-15)) break;
+  }
+  return x;
+}
+
+}  // namespace eig
+
+inline int rmhd_eigensystem(const double *U, const double *P, double *L, double *R, double *lam, int dim,
+                            double gamma)
+{
+  (void)U;
+  using namespace eig;
+  const int vn = dim == 1 ? vx : dim == 2 ? vy : vz;
+  const int bn = dim == 1 ? Bx : dim == 2 ? By : Bz;
+  // Jacobians in primitive variables by centred differences
+  double M[64], B[64];
+  {
+    for (int k = 0; k < 8; ++k) {
+      const double scale = (k == rho || k == pre) ? std::fabs(P[k]) : 1.0;
+      const double h = 1e-6 * (scale > 0.0 ? scale : 1.0);
+      double Pp[8], Pm[8], Up[8], Um[8], Fp[8], Fm[8];
+      for (int i = 0; i < 8; ++i) Pp[i] = Pm[i] = P[i];
+      Pp[k] += h;
+      Pm[k] -= h;
+      state_fluxes(Pp, Up, Fp, dim, gamma);
+      state_fluxes(Pm, Um, Fm, dim, gamma);
+      for (int i = 0; i < 8; ++i) {
+        M[i*8 + k] = (Up[i] - Um[i]) / (2.0*h);
+        B[i*8 + k] = (Fp[i] - Fm[i]) / (2.0*h);
+      }
+    }
+  }
+  double AP[64], Mc[64];
+  for (int i = 0; i < 64; ++i) { AP[i] = B[i]; Mc[i] = M[i]; }
+  if (solve(Mc, AP, 8, 8)) return 1;   // AP = M^-1 B
+  // eigenvalues
+  const double V2 = P[vx]*P[vx] + P[vy]*P[vy] + P[vz]*P[vz];
+  const double Bv = P[Bx]*P[vx] + P[By]*P[vy] + P[Bz]*P[vz];
+  const double B2 = P[Bx]*P[Bx] + P[By]*P[By] + P[Bz]*P[Bz];
+  const double W = 1.0 / std::sqrt(1.0 - V2);
+  const double b0 = W*Bv;
+  const double b2 = (B2 + b0*b0) / (W*W);
+  const double bi = (P[bn] + b0*W*P[vn]) / W;
+  const double v = P[vn];
+  const double rhoh = P[rho] + P[pre] / (gamma - 1.0) + P[pre];
+  const double cs2 = gamma*P[pre] / rhoh;
+  const double K = rhoh * (1.0/cs2 - 1.0) * W*W*W*W;
+  const double Lq = -(rhoh + b2/cs2) * W*W;
+  double ev[8];
+  ev[0] = 0.0;                                   // normal field
+  ev[1] = v;                                     // entropy
+  {
+    const double e = std::sqrt(rhoh + b2);       // Alfven: (b^n +- e u^n) / (b^0 +- e u^0)
+    ev[2] = (bi + e*W*v) / (b0 + e*W);
+    ev[3] = (bi - e*W*v) / (b0 - e*W);
+  }
+  {
+    const double ca2 = b2 / (rhoh + b2);
+    const double w2 = cs2 + ca2 - cs2*ca2;
+    const double den = 1.0 / (1.0 - V2*w2);
+    const double rt = std::sqrt(w2*(1.0 - V2)*(1.0 - V2*w2 - v*v*(1.0 - w2)));
+    const double lp = std::min((v*(1.0 - w2) + rt)*den + 1e-12, 1.0);
+    const double lm = std::max((v*(1.0 - w2) - rt)*den - 1e-12, -1.0);
+    const double r1 = quartic_root_outside(K, Lq, b0, bi, v, lp);
+    const double r2 = quartic_root_outside(K, Lq, b0, bi, v, lm);
+    // deflate A4 x^4 + ... + A0 by (x - r1)(x - r2): the slow magnetosonic pair
+    const double A4 = K - Lq - b0*b0;
+    const double A3 = -4*K*v + Lq*v*2 + 2*b0*bi;
+    const double A2 = 6*K*v*v + Lq*(1.0 - v*v) + b0*b0 - bi*bi;
+    const double s = r1 + r2, pr = r1*r2;
+    const double c1 = A3 + A4*s;
+    const double c0 = A2 + c1*s - A4*pr;
+    double disc = c1*c1 - 4.0*A4*c0;
+    if (disc < 0.0) disc = 0.0;
+    const double sq = std::sqrt(disc);
+    ev[4] = r1;
+    ev[5] = r2;
+    // the deflated pair, polished by Newton on the factored quartic
+    ev[6] = quartic_root_outside(K, Lq, b0, bi, v, (-c1 + sq) / (2.0*A4));
+    ev[7] = quartic_root_outside(K, Lq, b0, bi, v, (-c1 - sq) / (2.0*A4));
+  }
+  std::sort(ev, ev + 8);
+  // eigenvectors, cluster by cluster
+  double RP[64];
+  const double ctol = 1e-7;
+  for (int i = 0; i < 8;) {
+    int j = i + 1;
+    while (j < 8 && ev[j] - ev[j - 1] < ctol) ++j;
+    double mean = 0.0;
+    for (int k = i; k < j; ++k) mean += ev[k];
+    mean /= (j - i);
+    double A[64];
+    for (int r = 0; r < 8; ++r)
+      for (int c = 0; c < 8; ++c) A[r*8 + c] = AP[r*8 + c] - (r == c ? mean : 0.0);
+    if (null_space(A, RP, i, j - i)) return 2;
+    for (int k = i; k < j; ++k) ev[k] = mean;
+    i = j;
+  }
+  // conserved right eigenvectors, unit max-norm columns; L = R^-1
+  for (int r = 0; r < 8; ++r)
+    for (int c = 0; c < 8; ++c) {
+      double s = 0.0;
+      for (int k = 0; k < 8; ++k) s += M[r*8 + k] * RP[k*8 + c];
+      R[r*8 + c] = s;
+    }
+  for (int c = 0; c < 8; ++c) {
+    double m = 0.0;
+    for (int r = 0; r < 8; ++r) m = std::max(m, std::fabs(R[r*8 + c]));
+    if (!(m > 0.0)) return 1;
+    for (int r = 0; r < 8; ++r) R[r*8 + c] /= m;
+  }
+  double Rc[64];
+  for (int i = 0; i < 64; ++i) { Rc[i] = R[i]; L[i] = (i / 8 == i % 8) ? 1.0 : 0.0; }
+  if (solve(Rc, L, 8, 8)) return 1;
+  double lmax = 0.0, err = 0.0;
+  for (int r = 0; r < 8; ++r)
+    for (int c = 0; c < 8; ++c) {
+      lmax = std::max(lmax, std::fabs(L[r*8 + c]));
+      double s = 0.0;
+      for (int k = 0; k < 8; ++k) s += L[r*8 + k] * R[k*8 + c];
+      err = std::max(err, std::fabs(s - (r == c ? 1.0 : 0.0)));
+    }
+  if (!(lmax < 1e8) || !(err < 1e-8)) return 3;
+  for (int k = 0; k < 8; ++k) lam[k] = ev[k];
+  return 0;
+}
+
+}  // namespace oracle

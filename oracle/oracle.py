@@ -145,6 +145,8 @@ class Lib:
             L.oracle_count_flops.argtypes = [C.c_void_p, _dp, C.c_long, _dp]
             L.oracle_c2p_from_start_batch.argtypes = [C.c_long, _dp, _dp, _dp, C.c_double,
                                                       C.c_double, _ip, _ip]
+            L.oracle_eigensystem_batch.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _ip, C.c_int,
+                                                   C.c_double]
 
     def _f(self, name):
         return getattr(self.lib, self.p + name)
@@ -184,6 +186,17 @@ class Lib:
         self.lib.oracle_c2p_from_start_batch(len(U), _d(U), _d(start), _d(P), gamma, pfloor,
                                              _i(err), _i(it))
         return P, err, it
+
+    def eigensystem(self, P, dim, gamma):
+        """RMHD characteristic eigensystem (oracle only): L, R (n x 8 x 8), lam, ok."""
+        P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 8)
+        n = len(P)
+        L = np.zeros((n, 8, 8))
+        R = np.zeros((n, 8, 8))
+        lam = np.zeros((n, 8))
+        ok = np.zeros(n, dtype=np.int32)
+        self.lib.oracle_eigensystem_batch(n, _d(P), _d(L), _d(R), _d(lam), _i(ok), dim, gamma)
+        return L, R, lam, ok.astype(bool)
 
     def flux_eigen(self, P, dim, gamma):
         P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 8)

@@ -8,6 +8,7 @@
  * Function names and arguments match liboracle.so (prefix ref_ instead of oracle_).
  */
 #include "mara.hpp"
+#include "mara_oracle.hpp"   // rmhd_eigensystem: the RMHD eigensystem the reference lacks
 
 #include <cstring>
 #include <valarray>
@@ -49,15 +50,17 @@ static MaraApplication *App = nullptr;
 
 /* The reference's RMHD fluid with an Eigensystem: FluidEquations::Eigensystem
  * (Mara/hydro.hpp:146-147) is empty for it, so WenoSplit would read uninitialised
- * matrices. Mode 0 supplies the identity (component-wise splitting), which lets the
- * reference's own weno-split sweep run on RMHD. */
+ * matrices. Mode 0 supplies the identity (component-wise splitting), mode 1 the oracle's
+ * RMHD eigensystem (identity where that is ill-conditioned), which lets the reference's own
+ * weno-split sweep run on RMHD. */
 class RmhdWithEigensystem : public AdiabaticIdealRmhd {
 public:
   int mode = 0;
+  double gamma = 4.0 / 3.0;
   void Eigensystem(const double *U, const double *P, double *L, double *R, double *lam,
                    int dim) const override
   {
-    (void)U; (void)P; (void)dim;
+    if (mode == 1 && oracle::rmhd_eigensystem(U, P, L, R, lam, dim, gamma) == 0) return;
     for (int i = 0; i < 8; ++i) {
       lam[i] = 0.0;
       for (int j = 0; j < 8; ++j) L[i*8 + j] = R[i*8 + j] = (i == j) ? 1.0 : 0.0;
@@ -105,6 +108,7 @@ void *ref_create(const oracle_config *c)
     App->godunov = new WenoSplit;
     GodunovOperator::fluxsplit_method = GodunovOperator::FLUXSPLIT_LOCAL_LAX_FRIEDRICHS;
     static_cast<RmhdWithEigensystem *>(App->fluid)->mode = c->eigen;
+    static_cast<RmhdWithEigensystem *>(App->fluid)->gamma = c->gamma;
   }
   else App->godunov = new MethodOfLinesSplit;
   App->boundary = new PeriodicBoundary;

@@ -120,6 +120,10 @@ GRID_CASES = [
                                            advance="rk3"), 2),
     ("weno_split_rk2_superluminal_ereset", "superluminal",
      dict(godunov="weno-split", advance="rk2", ereset=True, preset=0.5, dreset=2.0), 2),
+    # ... with the characteristic projection: the reference's WenoSplit driven by the
+    # oracle's RMHD eigensystem
+    ("weno_split_char_rk3", "smooth", dict(godunov="weno-split", eigen="rmhd", advance="rk3"), 3),
+    ("weno_split_char_rk2_jump", "jump", dict(godunov="weno-split", eigen="rmhd", advance="rk2"), 3),
 ]
 
 
