@@ -172,6 +172,36 @@ def cpu_baseline(w, seconds_target=15.0):
                       f"{el:.1f} s single-threaded on the GPU box's host"}
 
 
+def weno_split_record(w, N, steps):
+    """The weno-split operator (local Lax-Friedrichs, finite-difference WENO5, RK3) on the
+    same turbulence at N^3: component-wise and RMHD-characteristic projections, FP64 and
+    mixed, timed over `steps` steps after the priming step and one warm-up step."""
+    base = workloads.replace(w.resized(N), godunov="weno-split", c2p="plain")
+    P0 = native.init_turbulence(**base.ic_kwargs())
+    rec = {"grid": N, "steps": steps, "fsplit": "llf", "extrap": "weno5 (finite difference)",
+           "advance": base.advance}
+    ref = None
+    for eigen in ("component", "rmhd"):
+        for prec in ("fp64", "mixed"):
+            log(f"[bench] weno-split {eigen} {prec} {N}^3")
+            wf = workloads.replace(base, eigen=eigen, precision=prec)
+            r = run_form(wf, P0, steps, 1, dts=None if ref is None else ref["dts"],
+                         total_steps=1 + steps)
+            if ref is None:
+                ref = r
+            st = r["stats"]
+            item = {"zone_updates_per_s": r["zone_updates_per_s"], "ms_per_step": r["ms_per_step"],
+                    "flux_ms_per_step": r["flux_ms_per_step"], "max_divB": r["max_divB"]}
+            if eigen == "rmhd":
+                stages = {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}[base.advance]
+                solves = 3 * stages * steps * (N + 1) * (N + 2 * base.Ng) ** 2
+                item["identity_projection_fraction"] = st.get("eig_identity", 0) / solves
+            if r is not ref:
+                item["L1_vs_component_fp64"] = l1_vs(r["P"], ref["P"], w.Ng)
+            rec[f"{eigen}_{prec}"] = item
+    return rec
+
+
 def traffic_record(w):
     path = os.path.join(ROOT, "profiles", f"traffic_{w.name}.json")
     if os.path.exists(path):
@@ -191,6 +221,9 @@ def main():
     ap.add_argument("--grid", type=int, default=None, help="override the workload's grid size")
     ap.add_argument("--forms", default="plain,mixed,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--weno-split-grid", type=int, default=128,
+                    help="grid of the weno-split measurement (0: skip it)")
+    ap.add_argument("--weno-split-steps", type=int, default=3)
     ap.add_argument("--model", default=os.path.join(ROOT, "ctf_amd", "models", "c2p_mlp.bin"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the "
@@ -279,6 +312,10 @@ def main():
             rec["mlp_fallback_fraction"] = st.get("mlp_fallbacks", 0) / g
         forms_out[name] = rec
 
+    ws = None
+    if args.weno_split_grid:
+        ws = weno_split_record(w, args.weno_split_grid, args.weno_split_steps)
+
     out = {
         "metric": METRIC,
         "value": plain["zone_updates_per_s"],
@@ -300,6 +337,8 @@ def main():
         "forms": forms_out,
         "roofline": roofline,
     }
+    if ws:
+        out["weno_split"] = ws
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline")
         cb = cpu_baseline(w)

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmaragpu.so")
 
 SOURCES = ["maragpu.hip", "mlp.hip", "mara_app.cpp", "initial_data.cpp"]
-HEADERS = ["rmhd_device.hpp", "mlp_device.hpp"]
+HEADERS = ["rmhd_device.hpp", "mlp_device.hpp", "eigensystem_device.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",

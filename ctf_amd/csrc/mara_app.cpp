@@ -104,7 +104,10 @@ int ensure_ctx()
   if (!a->have_domain) return fail("[Mara] need a domain to run this, use set_domain");
   if (!a->fluid_rmhd) return fail("[Mara] this build runs the 'rmhd' fluid only; use set_fluid('rmhd')");
   if (!a->have_godunov) return fail("[Mara] need a godunov operator, use set_godunov");
-  if (!a->have_riemann) return fail("[Mara] need a riemann solver, use set_riemann");
+  // WenoSplit splits the fluxes itself (Mara/weno-split.cpp:163-207); only plm-split calls
+  // the Riemann solver
+  if (!a->have_riemann && a->cfg.godunov != MARAGPU_WENO_SPLIT)
+    return fail("[Mara] need a riemann solver, use set_riemann");
   if (!a->have_advance) return fail("[Mara] need an advance option, use set_advance");
   if (!a->have_boundary) return fail("[Mara] need boundary conditions, use set_boundary");
   if (!a->dirty && a->ctx) return 0;
@@ -280,6 +283,14 @@ int mara_config_solver_str(const char *key, const char *value)
     if (!std::strcmp(value, "fp64")) c.precision = MARAGPU_FP64;
     else if (!std::strcmp(value, "mixed")) c.precision = MARAGPU_MIXED;
     else return fail(std::string("[Mara] no such precision '") + value + "'");
+    mark_dirty();
+    return 0;
+  }
+  if (std::strcmp(key, "eigensystem") == 0) {
+    // the weno-split projection; the reference's RMHD fluid has none (Mara/hydro.hpp:146-147)
+    if (!std::strcmp(value, "component")) c.eigen = MARAGPU_EIG_COMPONENT;
+    else if (!std::strcmp(value, "rmhd")) c.eigen = MARAGPU_EIG_RMHD;
+    else return fail(std::string("[Mara] no such eigensystem '") + value + "'");
     mark_dirty();
     return 0;
   }

@@ -33,6 +33,7 @@
 
 #include "rmhd_device.hpp"
 #include "mlp_device.hpp"
+#include "eigensystem_device.hpp"
 #include "../../include/maragpu.h"
 
 using namespace maragpu;
@@ -61,6 +62,7 @@ struct DevStats {
   unsigned long long redo_overflow;    // ... that did not fit the queue (FP32 HLLC kept)
   unsigned long long stage_failed[6];  // failed zones per c2p pass (ghost images included)
   unsigned long long stage_reset[6];   // zones reset by the emergency repair per pass (weno-split)
+  unsigned long long eig_identity;     // weno-split interfaces whose eigensystem was rejected
 };
 
 /* Mixed form: interfaces whose FP32 HLLD solve failed, recomputed in FP64 by
@@ -742,6 +744,89 @@ __global__ __launch_bounds__(kBlock) void k_flux_weno_split(const double *__rest
   }
 }
 
+/* The same sweep with the RMHD characteristic projection (eigen = RMHD): the eigensystem
+ * of the averaged primitive state 0.5 (P_z + P_z+S) in FP64 (L and R; the identity where
+ * it is rejected), the 6-zone split fluxes projected with L, the finite-difference WENO5
+ * of every characteristic field, and the projection of their sum back with R
+ * (Mara/weno-split.cpp:176-203, the eigensystem call at :181). In the mixed form (R =
+ * float) only the zone fluxes and speeds are FP32. */
+template <typename R, int DIM>
+__global__ __launch_bounds__(kBlock) void k_flux_weno_split_char(const double *__restrict__ Uw,
+                                                                 const double *__restrict__ P,
+                                                                 const R *__restrict__ FZ, const R *__restrict__ AZ,
+                                                                 R *__restrict__ Fout, GridDesc g, PhysParams pp,
+                                                                 DevStats *st)
+{
+#pragma clang fp contract(off)
+  const long c0 = DIM == 1 ? g.N[0] + 1 : g.n[0];
+  const long c1 = DIM == 2 ? g.N[1] + 1 : g.n[1];
+  const long c2 = DIM == 3 ? g.N[2] + 1 : g.n[2];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= c0 * c1 * c2) return;
+  const long k = t % c2 + (DIM == 3 ? g.Ng - 1 : 0);
+  const long j = (t / c2) % c1 + (DIM == 2 ? g.Ng - 1 : 0);
+  const long i = t / (c2 * c1) + (DIM == 1 ? g.Ng - 1 : 0);
+  const long T = g.T;
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  const long z = (i * g.n[1] + j) * g.n[2] + k;
+
+  double Lm[64], Rm[64], lam[8];
+  {
+    double Ph[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) Ph[q] = 0.5 * (P[q * T + z] + P[q * T + z + S]);
+    if (eig::rmhd_eigensystem<DIM>(Ph, Lm, Rm, lam, pp.gamma) != 0) {
+      eig::identity_eigensystem(Lm, Rm, lam);
+      atomicAdd(&st->eig_identity, 1ull);
+    }
+  }
+  R mlr = AZ[z - 2 * S];
+#pragma unroll
+  for (int m = -1; m <= 3; ++m) {
+    const R a = AZ[z + m * S];
+    mlr = a > mlr ? a : mlr;
+  }
+  // the projections and the reconstructions stay in FP64 also in the mixed form: the
+  // characteristic fields carry large parts that cancel on the way back through R, so
+  // their rounding is multiplied by the condition number of R
+  const double ml = (double)mlr;
+  double fp[8][6], fm[8][6];
+#pragma unroll 1
+  for (int m = 0; m < 6; ++m) {
+    const long zm = z + (m - 2) * S;
+    double Fp[8], Fm[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double F = (double)FZ[q * T + zm], U = Uw[q * T + zm];
+      Fp[q] = 0.5 * (F + ml * U);
+      Fm[q] = 0.5 * (F - ml * U);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        a += Lm[q * 8 + c] * Fp[c];
+        b += Lm[q * 8 + c] * Fm[c];
+      }
+      fp[q][m] = a;
+      fm[q][m] = b;
+    }
+  }
+  double f[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    f[q] = weno5<double, true, true>(fp[q][0], fp[q][1], fp[q][2], fp[q][3], fp[q][4], pp.IS, pp.sz10A) +
+           weno5<double, false, true>(fm[q][1], fm[q][2], fm[q][3], fm[q][4], fm[q][5], pp.IS, pp.sz10A);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    double a = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) a += Rm[q * 8 + c] * f[c];
+    Fout[q * T + z] = (R)a;
+  }
+}
+
 /* --------------------------------------------------------------------------
  * CT + divergence + RK stage (FP64, no contraction: the same roundings as the reference)
  * ------------------------------------------------------------------------ */
@@ -914,6 +999,21 @@ __global__ void kb_flux_eigen(long n, const double *P, double *F, double *ap, do
   am[i] = b;
 }
 
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void kb_eigensystem(long n, const double *P, double *L, double *R,
+                                                         double *lam, int *ok, double gamma)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[8], Lm[64], Rm[64], ev[8];
+  for (int q = 0; q < 8; ++q) p[q] = P[8 * i + q];
+  const int rc = eig::rmhd_eigensystem<DIM>(p, Lm, Rm, ev, gamma);
+  if (rc != 0) eig::identity_eigensystem(Lm, Rm, ev);
+  for (int q = 0; q < 64; ++q) { L[64 * i + q] = Lm[q]; R[64 * i + q] = Rm[q]; }
+  for (int q = 0; q < 8; ++q) lam[8 * i + q] = ev[q];
+  ok[i] = rc == 0;
+}
+
 __global__ void kb_reconstruct(long n, const double *V, double *out, int op, double theta, int IS,
                                double sz10A)
 {
@@ -1059,8 +1159,12 @@ void launch_weno_split(maragpu_ctx *c, const PhysParams &pp)
   k_zone_flux<R, DIM><<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Uw, c->Pw, FZ, AZ, g, pp, c->dstats);
   const long cnt = (long)(DIM == 1 ? g.N[0] + 1 : g.n[0]) * (DIM == 2 ? g.N[1] + 1 : g.n[1]) *
                    (DIM == 3 ? g.N[2] + 1 : g.n[2]);
-  k_flux_weno_split<R, R, DIM><<<nblocks(cnt), kBlock, 0, c->stream>>>(c->Uw, FZ, AZ,
-                                                                        static_cast<R *>(c->F[DIM - 1]), g, pp);
+  if (c->cfg.eigen == MARAGPU_EIG_RMHD)
+    k_flux_weno_split_char<R, DIM><<<nblocks(cnt), kBlock, 0, c->stream>>>(
+        c->Uw, c->Pw, FZ, AZ, static_cast<R *>(c->F[DIM - 1]), g, pp, c->dstats);
+  else
+    k_flux_weno_split<R, R, DIM><<<nblocks(cnt), kBlock, 0, c->stream>>>(c->Uw, FZ, AZ,
+                                                                          static_cast<R *>(c->F[DIM - 1]), g, pp);
 }
 
 struct EventPair { int a, b; };
@@ -1096,7 +1200,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
       cfg->recon > 2 || cfg->precision < 0 || cfg->precision > 1 || !(cfg->gamma > 1.0))
     return MARAGPU_E_INVALID;
   if (cfg->godunov != MARAGPU_PLM_SPLIT && cfg->godunov != MARAGPU_WENO_SPLIT) return MARAGPU_E_INVALID;
-  if (cfg->eigen != MARAGPU_EIG_COMPONENT) return MARAGPU_E_INVALID;
+  if (cfg->eigen != MARAGPU_EIG_COMPONENT && cfg->eigen != MARAGPU_EIG_RMHD) return MARAGPU_E_INVALID;
   if (cfg->godunov == MARAGPU_WENO_SPLIT && cfg->Ng < 3) return MARAGPU_E_INVALID;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MARAGPU_E_HIP;
@@ -1353,6 +1457,7 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     stats->mlp_fallbacks = (int64_t)h.mlp_fallbacks;
     stats->fp64_redo = (int64_t)h.redo;
     stats->fp64_redo_overflow = (int64_t)h.redo_overflow;
+    stats->eig_identity = (int64_t)h.eig_identity;
     float ms = 0.f;
     hipEventElapsedTime(&ms, c->ev[ev_start], c->ev[ev_end]);
     stats->device_ms = ms;
@@ -1495,6 +1600,29 @@ int maragpu_batch_flux_eigen(size_t n, const double *P, double *F, double *ap, d
   HIP_CHECK(hipMemcpy(F, dF.p, n * 64, hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(ap, dA.p, n * 8, hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(am, dB.p, n * 8, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+int maragpu_batch_eigensystem(size_t n, const double *P, double *L, double *R, double *lam, int *ok,
+                              int dim, double gamma)
+{
+  if (!P || !L || !R || !lam || !ok || dim < 1 || dim > 3) return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  DevBuf dP, dL, dR, dV, dK;
+  if (dalloc(dP, n * 64) || dalloc(dL, n * 512) || dalloc(dR, n * 512) || dalloc(dV, n * 64) ||
+      dalloc(dK, n * 4))
+    return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dP.p, P, n * 64, hipMemcpyHostToDevice));
+  auto *p = dP.get<double>(); auto *l = dL.get<double>(); auto *r = dR.get<double>();
+  auto *v = dV.get<double>(); auto *k = dK.get<int>();
+  if (dim == 1) kb_eigensystem<1><<<nblocks(n), kBlock>>>((long)n, p, l, r, v, k, gamma);
+  if (dim == 2) kb_eigensystem<2><<<nblocks(n), kBlock>>>((long)n, p, l, r, v, k, gamma);
+  if (dim == 3) kb_eigensystem<3><<<nblocks(n), kBlock>>>((long)n, p, l, r, v, k, gamma);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(L, dL.p, n * 512, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(R, dR.p, n * 512, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(lam, dV.p, n * 64, hipMemcpyDeviceToHost));
+  HIP_CHECK(hipMemcpy(ok, dK.p, n * 4, hipMemcpyDeviceToHost));
   return MARAGPU_OK;
 }
 

@@ -34,7 +34,7 @@ EXPORTS = [
     "maragpu_advance", "maragpu_get_timestep", "maragpu_measure_divB", "maragpu_kernel_times",
     "maragpu_strerror", "maragpu_batch_prim_to_cons", "maragpu_batch_cons_to_prim",
     "maragpu_batch_flux_eigen", "maragpu_batch_reconstruct", "maragpu_batch_riemann",
-    "mara_start", "mara_close", "mara_set_fluid", "mara_set_eos", "mara_set_boundary",
+    "maragpu_batch_eigensystem", "mara_start", "mara_close", "mara_set_fluid", "mara_set_eos", "mara_set_boundary",
     "mara_set_riemann", "mara_set_godunov", "mara_set_advance", "mara_config_solver_str",
     "mara_config_solver_num", "mara_config_solver_bool", "mara_set_domain", "mara_advance",
     "mara_get_timestep", "mara_set_primitive", "mara_measure_mean_max_divB", "mara_last_error",
@@ -70,7 +70,8 @@ class Stats(C.Structure):
                 ("c2p_failed", C.c_int64), ("hlld_c2p_iterations", C.c_int64),
                 ("hlld_newton_iterations", C.c_int64), ("mlp_guesses", C.c_int64),
                 ("mlp_fallbacks", C.c_int64), ("fp64_redo", C.c_int64),
-                ("fp64_redo_overflow", C.c_int64), ("device_ms", C.c_double)]
+                ("fp64_redo_overflow", C.c_int64), ("eig_identity", C.c_int64),
+                ("device_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -135,6 +136,7 @@ def lib():
                                                 C.c_double]),
         "maragpu_batch_riemann": (C.c_int, [_sz, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int,
                                             C.c_double, C.c_double, _ip, _dp]),
+        "maragpu_batch_eigensystem": (C.c_int, [_sz, _dp, _dp, _dp, _dp, _ip, C.c_int, C.c_double]),
         "mara_start": (C.c_int, []),
         "mara_close": (C.c_int, []),
         "mara_set_fluid": (C.c_int, [C.c_char_p]),
@@ -318,6 +320,19 @@ def flux_eigen(P, dim, gamma):
     am = np.zeros(len(P))
     check(lib().maragpu_batch_flux_eigen(len(P), dptr(P), dptr(F), dptr(ap), dptr(am), dim, gamma))
     return F, ap, am
+
+
+def eigensystem(P, dim, gamma):
+    """RMHD characteristic eigensystem of each primitive state: L, R (n, 8, 8), ascending
+    eigenvalues (n, 8) and ok (n,), False where the identity replaces a rejected one."""
+    P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 8)
+    n = len(P)
+    L = np.zeros((n, 8, 8))
+    R = np.zeros((n, 8, 8))
+    lam = np.zeros((n, 8))
+    ok = np.zeros(n, dtype=np.int32)
+    check(lib().maragpu_batch_eigensystem(n, dptr(P), dptr(L), dptr(R), dptr(lam), iptr(ok), dim, gamma))
+    return L, R, lam, ok.astype(bool)
 
 
 def reconstruct(V, op, theta=2.0, IS="js96", sz10A=50.0):

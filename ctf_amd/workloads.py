@@ -39,6 +39,8 @@ class Workload:
     seed: int = 20240601
     gamma: float = 4.0 / 3.0
     Ng: int = 3
+    godunov: str = "plm-split"
+    eigen: str = "component"
 
     def ic_kwargs(self, N=None):
         n = N or self.N
@@ -51,7 +53,8 @@ class Workload:
         n = N or self.N
         return dict(N=(n, n, n), Ng=self.Ng, gamma=self.gamma, extrap=self.extrap, theta=2.0,
                     IS="js96", riemann=self.riemann, advance=self.advance,
-                    precision=self.precision, c2p=self.c2p)
+                    precision=self.precision, c2p=self.c2p, godunov=self.godunov,
+                    eigen=self.eigen)
 
     def resized(self, N):
         return replace(self, N=N)

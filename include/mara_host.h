@@ -11,11 +11,13 @@
  *   mara_set_boundary                luaC_set_boundary               (:887-932)   'periodic'
  *   mara_set_riemann                 luaC_set_riemann                (:934-958)   'hll' 'hllc' 'hlld'
  *   mara_set_godunov                 luaC_set_godunov                (:960-984)   'plm-split'
+ *                                                                                 'weno-split'
  *   mara_set_advance                 luaC_set_advance                (:1140-1167) 'single' 'rk2' 'rk3' 'rk4'
  *   mara_config_solver_{str,num,bool} one key of luaC_config_solver  (:996-1138); extra keys
- *                                    'precision' = 'fp64' | 'mixed', 'c2p' = 'plain' | 'learned'
- *                                    and 'c2p_model' = path of the network file (default:
- *                                    models/c2p_mlp.bin next to the library)
+ *                                    'precision' = 'fp64' | 'mixed', 'c2p' = 'plain' | 'learned',
+ *                                    'c2p_model' = path of the network file (default:
+ *                                    models/c2p_mlp.bin next to the library) and
+ *                                    'eigensystem' = 'component' | 'rmhd' (weno-split projection)
  *   mara_set_domain                  luaC_set_domain                 (:726-755)
  *   mara_advance                     luaC_Mara_advance               (:297-341) -> kz/s, errors
  *   mara_get_timestep                luaC_Mara_get_timestep          (:374-381)

@@ -27,6 +27,9 @@
  *   maragpu_batch_reconstruct          reconstruct(v, op) of weno.h (Mara/weno.h:17-20, weno.c:58-71)
  *   maragpu_batch_riemann              RiemannSolver::IntercellFlux (Mara/hydro.hpp:217-218) of
  *                                      HllRiemannSolver, HllcRmhdRiemannSolver, HlldRmhdRiemannSolver
+ *   maragpu_batch_eigensystem          FluidEquations::Eigensystem (Mara/hydro.hpp:146-147), called
+ *                                      by WenoSplit (Mara/weno-split.cpp:181); the reference's RMHD
+ *                                      fluid leaves it empty, this is the RMHD characteristic one
  */
 #ifndef MARAGPU_H
 #define MARAGPU_H
@@ -72,7 +75,7 @@ typedef struct maragpu_config {
   int ereset;        /* emergency reset of failed zones */
   double preset, dreset;
   int godunov;       /* MARAGPU_PLM_SPLIT or MARAGPU_WENO_SPLIT (local Lax-Friedrichs splitting) */
-  int eigen;         /* weno-split projection: MARAGPU_EIG_COMPONENT */
+  int eigen;         /* weno-split projection: MARAGPU_EIG_COMPONENT or MARAGPU_EIG_RMHD */
 } maragpu_config;
 
 typedef struct maragpu_stats {
@@ -91,6 +94,8 @@ typedef struct maragpu_stats {
   int64_t mlp_fallbacks;        /* learned c2p: zones that fell back to the plain cascade */
   int64_t fp64_redo;            /* mixed form: interfaces whose FP32 HLLD failed, redone in FP64 */
   int64_t fp64_redo_overflow;   /* ... beyond the redo queue (FP32 HLLC flux kept) */
+  int64_t eig_identity;         /* weno-split, eigen = RMHD: interfaces projected with the
+                                   identity because the eigensystem was rejected */
   double device_ms;             /* device time of the step */
 } maragpu_stats;
 
@@ -124,6 +129,12 @@ int maragpu_batch_reconstruct(size_t n, const double *V, double *out, int op, do
                               double sz10A);
 int maragpu_batch_riemann(size_t n, int solver, int precision, const double *PL, const double *PR,
                           double *F, int dim, double gamma, double pfloor, int *err, double *lam);
+/* RMHD characteristic eigensystem of n primitive states for the flux along dim: L, R
+ * (64 doubles each per state, row-major, L R = I), ascending eigenvalues lam (8 per state),
+ * ok = 1 where the eigensystem is usable (else L = R = I and lam = 0, the projection the
+ * weno-split sweep then uses) */
+int maragpu_batch_eigensystem(size_t n, const double *P, double *L, double *R, double *lam, int *ok,
+                              int dim, double gamma);
 
 /* ---------------------------------------------------------------------------------
  * Learned c2p initial guess (no counterpart in the reference: the reference starts its

@@ -1101,13 +1101,14 @@ namespace oracle {
 /* ---------------------------------------------------------------------------------
  * RMHD characteristic eigensystem for the weno-split projection. No reference
  * counterpart: FluidEquations::Eigensystem (Mara/hydro.hpp:146-147) is empty for the RMHD
- * fluid. Pinned by invariants only (tests/test_eigensystem.py): A R = R diag(lam) for the
- * centred-difference Jacobian A = dF/dU, L R = I, the eigenvalues are the entropy speed,
+ * fluid. Pinned by invariants only (tests/test_eigensystem.py): A R = R diag(lam) for an
+ * independent centred-difference Jacobian A = dF/dU, L R = I, the eigenvalues are the entropy speed,
  * the two Alfven speeds, the four roots of FluxAndEigenvalues' quartic and zero (the
  * normal field), and for a uniform state the projected split fluxes reproduce the
  * component-wise ones.
  *
- * Method: M = dU/dP and B = dF/dP by centred differences; A_P = M^-1 B (similar to dF/dU);
+ * Method: M = dU/dP and B = dF/dP by forward-mode differentiation of PrimToCons and the
+ * flux formulas (exact to round-off); A_P = M^-1 B (similar to dF/dU);
  * eigenvalues in closed form where they have one and otherwise by Newton on the factored
  * quartic plus deflation; the eigenvectors of each eigenvalue cluster span the null space
  * of A_P - lam I (Gaussian elimination with complete pivoting); R = M R_P with unit
@@ -1116,10 +1117,51 @@ namespace oracle {
  * ------------------------------------------------------------------------------- */
 namespace eig {
 
-inline void state_fluxes(const double *P, double *U, double *F, int dim, double gamma)
+/* a value and its derivative along one primitive variable (forward-mode differentiation:
+ * every operation applies its derivative rule, so the Jacobians below are exact to
+ * round-off instead of carrying the cancellation of a difference quotient) */
+struct D1 { double v, d; };
+inline D1 operator+(D1 a, D1 b) { return {a.v + b.v, a.d + b.d}; }
+inline D1 operator-(D1 a, D1 b) { return {a.v - b.v, a.d - b.d}; }
+inline D1 operator*(D1 a, D1 b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+inline D1 operator/(D1 a, D1 b) { const double q = a.v / b.v; return {q, (a.d - q * b.d) / b.v}; }
+inline D1 operator+(double s, D1 a) { return {s + a.v, a.d}; }
+inline D1 operator-(double s, D1 a) { return {s - a.v, -a.d}; }
+inline D1 operator*(double s, D1 a) { return {s * a.v, s * a.d}; }
+inline D1 operator/(double s, D1 b) { const double q = s / b.v; return {q, -(q * b.d) / b.v}; }
+inline D1 operator/(D1 a, double s) { return {a.v / s, a.d / s}; }
+inline D1 sqrt(D1 a) { const double r = std::sqrt(a.v); return {r, a.d / (2.0 * r)}; }
+
+/* U and the flux along dim of the primitive state P: PrimToCons (Mara/rmhd.cpp:133-163)
+ * and the flux part of FluxAndEigenvalues (:164-199), in the reference's operation order;
+ * T = double gives the same values as prim_to_cons and flux_and_eigenvalues above */
+template <class T> inline void state_fluxes_t(const T *P, T *U, T *F, int dim, double gamma)
 {
-  prim_to_cons(P, U, gamma);
-  flux_and_eigenvalues<double>(U, P, F, nullptr, nullptr, dim, gamma);
+  using std::sqrt;
+  const T V2 = P[vx]*P[vx] + P[vy]*P[vy] + P[vz]*P[vz];
+  const T B2 = P[Bx]*P[Bx] + P[By]*P[By] + P[Bz]*P[Bz];
+  const T Bv = P[Bx]*P[vx] + P[By]*P[vy] + P[Bz]*P[vz];
+  const T W2 = 1.0 / (1.0 - V2);
+  const T W = sqrt(W2);
+  const T b0 = W*Bv;
+  const T b2 = (B2 + b0*b0) / W2;
+  const T b[3] = {(P[Bx] + b0 * W*P[vx]) / W, (P[By] + b0 * W*P[vy]) / W, (P[Bz] + b0 * W*P[vz]) / W};
+  const T e = (P[pre] / (gamma - 1)) / P[rho];
+  const T e_ = e + 0.5 * b2 / P[rho];
+  const T p_ = P[pre] + 0.5 * b2;
+  const T h_ = 1.0 + e_ + p_ / P[rho];
+  U[ddd] = P[rho] * W;
+  U[tau] = P[rho] * h_ * W2 - p_ - b0*b0 - U[ddd];
+  for (int a = 0; a < 3; ++a) U[Sx + a] = P[rho] * h_ * W2 * P[vx + a] - b0*b[a];
+  for (int a = 0; a < 3; ++a) U[Bx + a] = P[Bx + a];
+  const int vn = vx + dim - 1, bn = Bx + dim - 1;
+  F[ddd] = U[ddd] * P[vn];
+  F[tau] = U[tau] * P[vn] - b0*P[bn] / W + p_*P[vn];
+  for (int a = 0; a < 3; ++a) {
+    F[Sx + a] = U[Sx + a] * P[vn] - b[a]*P[bn] / W;
+    if (a == dim - 1) F[Sx + a] = F[Sx + a] + p_;
+    F[Bx + a] = (a == dim - 1) ? T{} : P[vn]*P[Bx + a] - P[vx + a]*P[bn];
+  }
 }
 
 /* solves A X = Bm in place (A n x n, Bm n x m, row-major) by partial pivoting; 0 on success */
@@ -1214,22 +1256,16 @@ inline int rmhd_eigensystem(const double *U, const double *P, double *L, double 
   using namespace eig;
   const int vn = dim == 1 ? vx : dim == 2 ? vy : vz;
   const int bn = dim == 1 ? Bx : dim == 2 ? By : Bz;
-  // Jacobians in primitive variables by centred differences
+  // Jacobians in primitive variables, M = dU/dP and B = dF/dP, column k by
+  // forward-mode differentiation along P[k]
   double M[64], B[64];
-  {
-    for (int k = 0; k < 8; ++k) {
-      const double scale = (k == rho || k == pre) ? std::fabs(P[k]) : 1.0;
-      const double h = 1e-6 * (scale > 0.0 ? scale : 1.0);
-      double Pp[8], Pm[8], Up[8], Um[8], Fp[8], Fm[8];
-      for (int i = 0; i < 8; ++i) Pp[i] = Pm[i] = P[i];
-      Pp[k] += h;
-      Pm[k] -= h;
-      state_fluxes(Pp, Up, Fp, dim, gamma);
-      state_fluxes(Pm, Um, Fm, dim, gamma);
-      for (int i = 0; i < 8; ++i) {
-        M[i*8 + k] = (Up[i] - Um[i]) / (2.0*h);
-        B[i*8 + k] = (Fp[i] - Fm[i]) / (2.0*h);
-      }
+  for (int k = 0; k < 8; ++k) {
+    D1 Pd[8], Ud[8], Fd[8];
+    for (int i = 0; i < 8; ++i) Pd[i] = D1{P[i], i == k ? 1.0 : 0.0};
+    state_fluxes_t(Pd, Ud, Fd, dim, gamma);
+    for (int i = 0; i < 8; ++i) {
+      M[i*8 + k] = Ud[i].d;
+      B[i*8 + k] = Fd[i].d;
     }
   }
   double AP[64], Mc[64];

@@ -222,15 +222,18 @@ def test_turbulence_steps_vs_oracle(gpu, oracle_lib, extrap, riemann, advance, N
     compare_states(ctx.download(), Po)
 
 
+@pytest.mark.parametrize("eigen", ["component", "rmhd"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_weno_split_steps_vs_oracle(gpu, oracle_lib, precision):
-    """weno-split (local Lax-Friedrichs, component-wise) on seeded turbulence against the
-    oracle, which is pinned bitwise to the reference's own WenoSplit. FP64: the plain-form
-    tolerances; mixed (FP32 zone fluxes and WENO, FP64 c2p/RK/CT): L1 <= 1e-6."""
-    N = 20
+def test_weno_split_steps_vs_oracle(gpu, oracle_lib, precision, eigen):
+    """weno-split (local Lax-Friedrichs) on seeded turbulence against the oracle, which is
+    pinned bitwise to the reference's own WenoSplit, with the component-wise projection and
+    with the RMHD characteristic one. FP64: the plain-form tolerances; mixed (FP32 zone
+    fluxes and speeds, component-wise WENO in FP32; FP64 eigensystem, characteristic
+    projections and WENO, c2p, RK, CT): L1 <= 1e-6."""
+    N = 20 if eigen == "component" else 16
     P0 = gpu.init_turbulence((N, N, N), v_rms=0.4, v_kmax=4.0, four_velocity=True, dB_rms=0.3,
                              A_kmax=3.0, seed=7)
-    s = Setup(N=(N, N, N), godunov="weno-split", advance="rk3")
+    s = Setup(N=(N, N, N), godunov="weno-split", advance="rk3", eigen=eigen)
     so = oracle_lib.solver(s)
     ctx = gpu.Context(setup_to_config(gpu, s, precision=precision))
     ctx.upload(P0)
@@ -341,4 +344,31 @@ def test_mara_verbs_drive_the_device(gpu, oracle_lib):
     assert abs(mx) < 1e-12
     with pytest.raises(MaraError):
         Mara.advance(P[:-1].copy(), dt)       # wrong size, as luaL_error in the reference
+    Mara.close()
+
+
+def test_mara_verbs_weno_split_characteristic(gpu, oracle_lib):
+    """set_godunov('weno-split') with config_solver{eigensystem = 'rmhd'} through the Lua
+    verbs: the same steps as the oracle's characteristic weno-split."""
+    from ctf_amd.mara import Mara
+    N = 12
+    Mara.start()
+    Mara.set_fluid("rmhd")
+    Mara.set_eos("gamma-law", GAMMA)
+    Mara.set_advance("rk3")
+    Mara.set_godunov("weno-split")
+    Mara.set_boundary("periodic")
+    Mara.config_solver({"fsplit": "llf", "IS": "js96", "eigensystem": "rmhd"}, True)
+    Mara.set_domain((0, 0, 0), (1, 1, 1), (N, N, N), 8, 3)
+    P = gpu.init_turbulence((N, N, N), v_rms=0.3, dB_rms=0.3, seed=5)
+    Po = P.copy()
+    so = oracle_lib.solver(Setup(N=(N, N, N), godunov="weno-split", advance="rk3", eigen="rmhd"))
+    dt = 0.0
+    for _ in range(3):
+        kz, ne = Mara.advance(P, dt)
+        eo, _ = so.advance(Po, dt)
+        assert ne == eo == 0 and kz > 0
+        dt = Mara.get_timestep(0.4)
+        assert abs(dt - so.get_timestep(0.4)) <= 1e-12 * dt
+    compare_states(P, Po)
     Mara.close()

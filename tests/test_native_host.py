@@ -103,6 +103,10 @@ def test_lua_verbs_reject_like_mara():
         Mara.config_solver({"fsplit": "marq"}, True)
     Mara.set_godunov("weno-split")
     Mara.config_solver({"fsplit": "llf"}, True)
+    with pytest.raises(MaraError, match="no such eigensystem"):
+        Mara.config_solver({"eigensystem": "roe"}, True)
+    Mara.config_solver({"eigensystem": "rmhd"}, True)
+    Mara.config_solver({"eigensystem": "component"}, True)
     Mara.config_solver({"unknown_key": 3.0, "precision": "mixed", "c2p": "plain"}, True)
     with pytest.raises(MaraError, match="need a domain"):
         Mara.advance(np.zeros(8), 0.0)
