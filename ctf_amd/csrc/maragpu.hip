@@ -41,6 +41,12 @@ using namespace maragpu;
 namespace {
 
 constexpr int kBlock = 256;
+// occupancy targets of k_flux_hlld_fast (waves per SIMD), measured at 256^3 WENO5/HLLD:
+// FP64 at 2 (209 VGPRs, 40 B/lane scratch; 4.0 ms, against 4.7 at 3 with 184 B spilled),
+// FP32 at 5 (96 VGPRs, 44 B/lane; 2.00 ms, against 2.21 at 4 and 2.18 at 6)
+constexpr int kFastFluxWaves64 = 2;
+constexpr int kFastFluxWaves32 = 5;
+constexpr int kRedoBlocks = 1024;    // grid of k_flux_redo (a grid-stride loop over its queue)
 // learned c2p: network and polish in one kernel (true) or as two passes over the grid
 constexpr bool kFusedLearnedC2p = true;
 
@@ -65,12 +71,13 @@ struct DevStats {
   unsigned long long eig_identity;     // weno-split interfaces whose eigensystem was rejected
 };
 
-/* Mixed form: interfaces whose FP32 HLLD solve failed, recomputed in FP64 by
- * k_flux_redo before the update reads the fluxes. */
+/* Interfaces of an HLLD sweep handed to the full FP64 solver (k_flux_redo) before the
+ * update reads the fluxes: in the mixed form those whose FP32 HLLD solve failed, in the
+ * plain form those outside the common path of the lean sweep (k_flux_hlld_fast). */
 struct RedoQueue {
   unsigned int *count;     // device counter, cleared before each sweep
   unsigned int cap;        // slots
-  long *z;                 // device slots: interface index z (left zone)
+  unsigned int *z;         // device slots: interface index z (left zone; T < 2^32)
 };
 
 struct GridDesc {
@@ -123,6 +130,24 @@ __device__ __forceinline__ void block_add(const unsigned long long (&v)[NV], uns
     for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
     if (s) atomicAdd(dst[threadIdx.x], s);
   }
+}
+
+/* Appends z for every lane of the wave with push set, with one atomic per wave; every
+ * lane of the wave calls it. Returns false on a lane whose slot lies past the capacity. */
+__device__ __forceinline__ bool queue_push(const RedoQueue &rq, bool push, long z)
+{
+  const unsigned long long m = __ballot(push);
+  if (m == 0ull) return true;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  unsigned int base = 0u;
+  if (lane == leader) base = atomicAdd(rq.count, (unsigned int)__popcll(m));
+  base = (unsigned int)__shfl((int)base, leader);
+  if (!push) return true;
+  const unsigned int k = base + (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+  if (k >= rq.cap) return false;
+  rq.z[k] = (unsigned int)z;
+  return true;
 }
 
 __device__ __forceinline__ void block_max_lambda(double lam, unsigned long long *dst)
@@ -577,13 +602,41 @@ __device__ __forceinline__ int interface_flux(const double *__restrict__ P, long
   return err;
 }
 
+/* The common path of an HLLD interface: reconstruction and riemann_hlld in FAST mode.
+ * Returns kHlldDefer, with F unwritten, where the full solver is needed. */
+template <typename R, int DIM>
+__device__ __forceinline__ int interface_flux_fast(const double *__restrict__ P, long T, long S, long z,
+                                                   int recon, const PhysParams &pp, R *F, R &lam,
+                                                   FluxCounters &fc)
+{
+  R ql[8], qr[8];
+  const R theta = (R)pp.theta;
+  for (int q = 0; q < 8; ++q) {
+    const double *Pq = P + q * T + z;
+    const R v0 = (R)Pq[0], v1 = (R)Pq[S];
+    if (recon == RECON_WENO5) {
+      const R vm2 = (R)Pq[-2 * S], vm1 = (R)Pq[-S], v2 = (R)Pq[2 * S], v3 = (R)Pq[3 * S];
+      ql[q] = weno5<R, true>(vm2, vm1, v0, v1, v2, pp.IS, (R)pp.sz10A);
+      qr[q] = weno5<R, false>(vm1, v0, v1, v2, v3, pp.IS, (R)pp.sz10A);
+    }
+    else if (recon == RECON_PLM) {
+      ql[q] = plm<R>((R)Pq[-S], v0, v1, R(+1.0), theta);
+      qr[q] = plm<R>(v0, v1, (R)Pq[2 * S], R(-1.0), theta);
+    }
+    else {
+      ql[q] = v0;
+      qr[q] = v1;
+    }
+  }
+  return riemann_hlld<R, DIM, true>(ql, qr, F, (R)pp.gamma, pp.pfloor, lam, fc);
+}
+
 /* Flux sweep along DIM, one thread per interface. With a redo queue (mixed form, HLLD),
  * an interface whose FP32 HLLD solve fell back to HLLC or failed is queued for
  * k_flux_redo instead; its FP32 result stays only if the queue is full. */
-/* Occupancy targets (waves per SIMD), measured at 256^3 WENO5/HLLD: FP32 sweeps run
- * best at 3 (168 VGPRs, little spilling; 3.06 -> 2.26 ms), FP64 sweeps at 2 even though
- * the register budget of 256 then spills (7.2 -> 6.2 ms: the spill traffic costs less
- * than the latency that a single wave cannot hide). */
+/* Occupancy targets (waves per SIMD) of the full sweep, which runs HLL and HLLC: FP32 at
+ * 3, FP64 at 2 even where 256 registers spill (the spill traffic costs less than the
+ * latency that a single wave cannot hide). */
 template <typename R, typename FT, int DIM, int SOLVER>
 __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 2) void k_flux(const double *__restrict__ P, FT *__restrict__ Fout,
                                                  GridDesc g, int recon, PhysParams pp, DevStats *st,
@@ -594,32 +647,27 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 2) void k_flux(const d
   const long z = 2 * S + (long)blockIdx.x * blockDim.x + threadIdx.x;
   double lam_out = 0.0;
   unsigned long long cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  bool want = false;
   if (z < T - 3 * S) {
     R lam = R(0);
     FluxCounters fc = {0, 0, 0, 0};
     R F[8];
     int retried;
     const int err = interface_flux<R, DIM, SOLVER>(P, T, S, z, recon, pp, F, lam, fc, retried);
-    bool queued = false;
-    if (rq.z && (fc.hlld_fail || err)) {
-      const unsigned int k = atomicAdd(rq.count, 1u);
-      if (k < rq.cap) {
-        rq.z[k] = z;
-        queued = true;
-      }
-      else cnt[6] = 1;
-    }
+    want = rq.z && (fc.hlld_fail || err);
 #pragma unroll
     for (int q = 0; q < 8; ++q) Fout[q * T + z] = (FT)F[q];
     lam_out = (double)lam;
-    if (!queued) {
-      cnt[0] = (unsigned long long)fc.hlld_ok;
-      cnt[1] = (unsigned long long)fc.hlld_fail;
-      cnt[2] = (unsigned long long)retried;
-      cnt[3] = err ? 1ull : 0ull;
-    }
+    cnt[0] = (unsigned long long)fc.hlld_ok;
+    cnt[1] = (unsigned long long)fc.hlld_fail;
+    cnt[2] = (unsigned long long)retried;
+    cnt[3] = err ? 1ull : 0ull;
     cnt[4] = (unsigned long long)fc.c2p_iters;
     cnt[5] = (unsigned long long)fc.newton_iters;
+  }
+  if (rq.z) {
+    if (!queue_push(rq, want, z)) cnt[6] = 1;     // full: the FP32 result stays
+    else if (want) cnt[0] = cnt[1] = cnt[2] = cnt[3] = 0;   // counted by the redo
   }
   block_max_lambda(lam_out, &st->lam_bits);
   unsigned long long *const dst[7] = { &st->hlld_success, &st->hlld_failure, &st->retries,
@@ -628,37 +676,74 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 2) void k_flux(const d
   block_add<7>(cnt, dst);
 }
 
-/* The queued interfaces of a mixed-form sweep, recomputed as the plain FP64 method does
- * (reconstruction, HLLD attempt ladder, HLLC fallback, retry) and written over the FP32
- * fluxes. One thread per queue slot; slots past the count exit. */
-template <int DIM, int SOLVER>
-__global__ __launch_bounds__(kBlock) void k_flux_redo(const double *__restrict__ P, float *__restrict__ Fout,
+/* HLLD sweep along DIM, common path only (interface_flux_fast): interfaces that need
+ * the c2p cascade, a later Newton attempt, the small-normal-field branch, the HLLC
+ * fallback or the retry are queued for k_flux_redo, which computes them with the full
+ * FP64 solver. In the plain form that is the result the full sweep gives; in the mixed
+ * form (R = float) the rare branches run in FP64 instead of FP32. Leaving them out of this
+ * kernel lowers its register peak. The queue holds every interface of the sweep, so it
+ * cannot overflow. */
+template <typename R, int DIM>
+__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? kFastFluxWaves32 : kFastFluxWaves64)
+void k_flux_hlld_fast(const double *__restrict__ P, R *__restrict__ Fout, GridDesc g, int recon, PhysParams pp,
+                      DevStats *st, RedoQueue rq)
+{
+  const long T = g.T;
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  const long z = 2 * S + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double lam_out = 0.0;
+  unsigned long long cnt[3] = {0, 0, 0};
+  bool defer = false;
+  if (z < T - 3 * S) {
+    R lam = R(0);
+    FluxCounters fc = {0, 0, 0, 0};
+    R F[8];
+    defer = interface_flux_fast<R, DIM>(P, T, S, z, recon, pp, F, lam, fc) != 0;
+    if (!defer) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Fout[q * T + z] = F[q];
+      cnt[0] = 1;
+      cnt[1] = (unsigned long long)fc.c2p_iters;
+      cnt[2] = (unsigned long long)fc.newton_iters;
+    }
+    lam_out = (double)lam;
+  }
+  queue_push(rq, defer, z);
+  block_max_lambda(lam_out, &st->lam_bits);
+  unsigned long long *const dst[3] = { &st->hlld_success, &st->hlld_c2p_iters, &st->hlld_newton_iters };
+  block_add<3>(cnt, dst);
+}
+
+/* The queued interfaces of a sweep, recomputed by the full FP64 method (reconstruction,
+ * HLLD attempt ladder, HLLC fallback, retry) and written over the sweep's fluxes (FP32
+ * in the mixed form, FP64 in the plain one). A grid-stride loop over the queue count. */
+template <typename FT, int DIM, int SOLVER>
+__global__ __launch_bounds__(kBlock) void k_flux_redo(const double *__restrict__ P, FT *__restrict__ Fout,
                                                       GridDesc g, int recon, PhysParams pp, DevStats *st,
                                                       RedoQueue rq)
 {
-  const unsigned int n = *rq.count;
-  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned int n = min(*rq.count, rq.cap);
   double lam_out = 0.0;
   unsigned long long cnt[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (k < n && k < rq.cap) {
+  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const long T = g.T;
     const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
-    const long z = rq.z[k];
+    const long z = (long)rq.z[k];
     double lam = 0.0;
     FluxCounters fc = {0, 0, 0, 0};
     double F[8];
     int retried;
     const int err = interface_flux<double, DIM, SOLVER>(P, T, S, z, recon, pp, F, lam, fc, retried);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) Fout[q * T + z] = (float)F[q];
-    lam_out = lam;
-    cnt[0] = (unsigned long long)fc.hlld_ok;
-    cnt[1] = (unsigned long long)fc.hlld_fail;
-    cnt[2] = (unsigned long long)retried;
-    cnt[3] = err ? 1ull : 0ull;
-    cnt[4] = (unsigned long long)fc.c2p_iters;
-    cnt[5] = (unsigned long long)fc.newton_iters;
-    cnt[6] = 1;
+    for (int q = 0; q < 8; ++q) Fout[q * T + z] = (FT)F[q];
+    lam_out = fmax(lam_out, lam);
+    cnt[0] += (unsigned long long)fc.hlld_ok;
+    cnt[1] += (unsigned long long)fc.hlld_fail;
+    cnt[2] += (unsigned long long)retried;
+    cnt[3] += err ? 1ull : 0ull;
+    cnt[4] += (unsigned long long)fc.c2p_iters;
+    cnt[5] += (unsigned long long)fc.newton_iters;
+    cnt[6] += 1;
   }
   block_max_lambda(lam_out, &st->lam_bits);
   unsigned long long *const dst[7] = { &st->hlld_success, &st->hlld_failure, &st->retries,
@@ -1135,16 +1220,13 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
     k_flux<R, FT, DIM, RIEMANN_HLLC><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, none);
     break;
   default:
-    if constexpr (sizeof(R) == 4) {
-      // mixed form: FP32 sweep, then the interfaces whose FP32 HLLD failed redone in FP64
-      hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
-      k_flux<R, FT, DIM, RIEMANN_HLLD><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, c->redo);
-      k_flux_redo<DIM, RIEMANN_HLLD><<<(c->redo.cap + kBlock - 1) / kBlock, kBlock, 0, c->stream>>>(
-          c->Pw, reinterpret_cast<float *>(F), c->g, c->cfg.recon, pp, c->dstats, c->redo);
-    }
-    else {
-      k_flux<R, FT, DIM, RIEMANN_HLLD><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, none);
-    }
+    // the sweep, then its queued interfaces by the full FP64 solver: in the mixed form the
+    // FP32 HLLD failures, in the plain form whatever left the common path
+    hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
+    k_flux_hlld_fast<R, DIM><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats,
+                                                                    c->redo);
+    k_flux_redo<FT, DIM, RIEMANN_HLLD><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp,
+                                                                              c->dstats, c->redo);
     break;
   }
 }
@@ -1209,6 +1291,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (!c) return MARAGPU_E_NOMEM;
   c->cfg = *cfg;
   c->g = make_grid(*cfg);
+  if (c->g.T >= (1L << 32)) { delete c; return MARAGPU_E_INVALID; }   // 32-bit queue slots
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return MARAGPU_E_HIP; }
   const size_t nd = 8 * (size_t)c->g.T;
   const size_t fbytes = nd * (cfg->precision == MARAGPU_MIXED ? sizeof(float) : sizeof(double));
@@ -1231,13 +1314,12 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
         hipMalloc(&c->FZ, (size_t)9 * c->g.T * rb) != hipSuccess)
       return fail_free(c);
   }
-  if (cfg->precision == MARAGPU_MIXED && cfg->riemann == MARAGPU_HLLD && cfg->godunov != MARAGPU_WENO_SPLIT) {
-    // room for 1 interface in 64 per sweep: FP32 HLLD fails on about 1 in 10^5 interfaces
-    // at Lorentz factors ~ 1.5 and 1 in 300 at ~ 10 (8 bytes a slot: 0.2 % of a state array)
-    const long cap = c->g.T / 64 > 4096 ? c->g.T / 64 : 4096;
+  if (cfg->riemann == MARAGPU_HLLD && cfg->godunov != MARAGPU_WENO_SPLIT) {
+    // a slot for every interface of a sweep (4 bytes a slot: half a state plane)
+    const long cap = c->g.T;
     c->redo.cap = (unsigned int)cap;
     if (hipMalloc(&c->redo.count, sizeof(unsigned int)) != hipSuccess ||
-        hipMalloc(&c->redo.z, (size_t)cap * sizeof(long)) != hipSuccess)
+        hipMalloc(&c->redo.z, (size_t)cap * sizeof(unsigned int)) != hipSuccess)
       return fail_free(c);
   }
   // fluxes outside the sweep ranges are never written; keep them defined

@@ -34,7 +34,10 @@ enum { RIEMANN_HLL = 0, RIEMANN_HLLC = 1, RIEMANN_HLLD = 2 };
  * FP32 scales the ones that sit below single-precision resolution. */
 template <typename R> struct Tol;
 template <> struct Tol<double> {
-  static constexpr bool nested_c2p_from_average = false;   // the reference's cascade order
+  // the c2p of the HLL state tries anton2dzw from the average of the two states (the
+  // cascade's second guess) before the cascade: the same root to round-off, and the same
+  // success set, since the full cascade runs whenever that attempt fails
+  static constexpr bool nested_c2p_from_average = true;
   static constexpr double newton_ftol = 1e-12;   // NewtonRaphesonSolver(15, 1e-12), riemann_hlld-rmhd.cpp:220
   static constexpr double fd_dp = 1e-8;          // HlldEquation48::Jacobian, :313
   static constexpr bool fd_relative = false;
@@ -1095,10 +1098,17 @@ struct FluxCounters {
  * Live-state economy: the jump states across the fast waves are formed before the
  * nested c2p, after which the outer U and F are dead; only the primitive states are
  * kept (to rebuild the flux side at the end and for the HLLC fallback, which averages
- * the normal field exactly as here, riemann_hllc-rmhd.cpp:60). */
-template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f,
-                                                                       R gamma, double pfloor, R &lam,
-                                                                       FluxCounters &cnt)
+ * the normal field exactly as here, riemann_hllc-rmhd.cpp:60).
+ * FAST = true is the common path alone: the nested c2p from the average state and the
+ * first Newton attempt from EstimateSolution(0). Anything else (an invalid state, the
+ * c2p cascade, the small-normal-field branch, attempts 2 and 3, the HLLC fallback)
+ * returns kHlldDefer without writing f; the caller hands the interface to the full
+ * solver, which then takes the same steps up to that point and gives the same result. */
+constexpr int kHlldDefer = 100;   // FAST mode: the interface needs the full solver
+
+template <typename R, int DIM, bool FAST = false>
+__device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f, R gamma, double pfloor, R &lam,
+                                            FluxCounters &cnt)
 {
   using T = Tol<R>;
   using Pm = Perm<DIM>;
@@ -1118,8 +1128,8 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
   {
     R Ul[8], Ur[8], Fl[8], Fr[8];
     const R gm1 = gamma - R(1);
-    if (prim_to_cons(Pl, Ul, gm1)) return 1;
-    if (prim_to_cons(Pr, Ur, gm1)) return 1;
+    if (prim_to_cons(Pl, Ul, gm1)) return FAST ? kHlldDefer : 1;
+    if (prim_to_cons(Pr, Ur, gm1)) return FAST ? kHlldDefer : 1;
     R epl, epr, eml, emr;
     flux_eig_pair<R, 1>(Pl, Pr, Ul, Ur, Fl, Fr, epl, eml, epr, emr, gamma);
     const R am = (eml < emr) ? eml : emr;
@@ -1172,8 +1182,10 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
       }
     }
   }
-  if (hll_c2p_failed)
+  if (hll_c2p_failed) {
+    if constexpr (FAST) return kHlldDefer;
     hll_c2p_failed = cons_to_prim<Shape<R>::unroll>(Uh, Ph, (double)gamma, pfloor, true, cnt.c2p_iters, c2p_first);
+  }
 
   bool success = false;
   R Fx[8];
@@ -1192,8 +1204,14 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
       est0 = (R)(P[pre] + 0.5*b2);
     }
     const bool small_bn = dfabs(Pl[Bx]) < T::small_bn || dfabs(Pr[Bx]) < T::small_bn;
+    if constexpr (FAST) {
+      if (small_bn) return kHlldDefer;
+      R p_star = est0;
+      success = e.newton(p_star, cnt.newton_iters) && e.flux(p_star, Pl, Pr, gamma, Fx);
+      if (!success) return kHlldDefer;
+    }
 #pragma unroll Shape<R>::unroll
-    for (int attempt = 0; attempt < 3 && !success; ++attempt) {
+    for (int attempt = 0; attempt < (FAST ? 0 : 3) && !success; ++attempt) {
       R p_star;
       if (small_bn || attempt == 1) {
         // EstimateSolution(1), eqn (55), :558-574 (the += / -= pair is kept: it rounds)
@@ -1227,6 +1245,7 @@ template <typename R, int DIM> __device__ __forceinline__ int riemann_hlld(const
       return 0;
     }
   }
+  if constexpr (FAST) return kHlldDefer;
   ++cnt.hlld_fail;
   // HLLC of the same (normal-field-averaged) states in the lab frame
   R ql[8], qr[8];
