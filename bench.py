@@ -279,7 +279,7 @@ def main():
     achieved = flops_per_launch / avg_flux_s / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                "kernel": "k_flux<double,WENO5,HLLD> (one axis sweep)",
+                "kernel": "k_flux_hlld_fast<double> + k_flux_redo (one axis sweep, WENO5/HLLD)",
                 "flops_per_launch": flops_per_launch,
                 "avg_launch_ms": plain["flux_ms_per_launch"],
                 "traffic": None}
