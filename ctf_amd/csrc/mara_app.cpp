@@ -32,6 +32,11 @@ struct App {
   std::string mlp_path;            // config_solver key 'c2p_model'; default models/c2p_mlp.bin
   bool dirty = true;
   double last_max_lambda = 0.0;
+  double units[3] = {1.0, 1.0, 1.0};   // PhysicalUnits(Length, Mass, Time): cgs, hydro.cpp:20
+  // Mara->PrimitiveArray: the array of the last set_primitive, which the measures read
+  // (advance does not change it, Mara/mara.cpp:297-341)
+  std::vector<double> prim;
+  bool prim_on_device = false;         // the context's committed state holds `prim`
   std::string err;
 
   App()
@@ -98,22 +103,28 @@ int load_mlp(App *a)
   return 0;
 }
 
-int ensure_ctx()
+/* The device context for the current configuration. An advance needs the whole solver
+ * set up; a measure only the domain and the fluid (the solver fields keep their
+ * defaults). */
+int ensure_ctx(bool for_advance = true)
 {
   App *a = app();
   if (!a->have_domain) return fail("[Mara] need a domain to run this, use set_domain");
   if (!a->fluid_rmhd) return fail("[Mara] this build runs the 'rmhd' fluid only; use set_fluid('rmhd')");
-  if (!a->have_godunov) return fail("[Mara] need a godunov operator, use set_godunov");
-  // WenoSplit splits the fluxes itself (Mara/weno-split.cpp:163-207); only plm-split calls
-  // the Riemann solver
-  if (!a->have_riemann && a->cfg.godunov != MARAGPU_WENO_SPLIT)
-    return fail("[Mara] need a riemann solver, use set_riemann");
-  if (!a->have_advance) return fail("[Mara] need an advance option, use set_advance");
-  if (!a->have_boundary) return fail("[Mara] need boundary conditions, use set_boundary");
+  if (for_advance) {
+    if (!a->have_godunov) return fail("[Mara] need a godunov operator, use set_godunov");
+    // WenoSplit splits the fluxes itself (Mara/weno-split.cpp:163-207); only plm-split calls
+    // the Riemann solver
+    if (!a->have_riemann && a->cfg.godunov != MARAGPU_WENO_SPLIT)
+      return fail("[Mara] need a riemann solver, use set_riemann");
+    if (!a->have_advance) return fail("[Mara] need an advance option, use set_advance");
+    if (!a->have_boundary) return fail("[Mara] need boundary conditions, use set_boundary");
+  }
   if (!a->dirty && a->ctx) return 0;
   if (a->ctx) {
     maragpu_destroy(a->ctx);
     a->ctx = nullptr;
+    a->prim_on_device = false;
   }
   int rc = maragpu_create(&a->cfg, &a->ctx);
   if (rc) return fail(std::string("[Mara] device context: ") + maragpu_strerror(rc), rc);
@@ -354,6 +365,8 @@ int mara_set_domain(const double *x0, const double *x1, const int *N, int Nd, in
   }
   c.Ng = Ng;
   app()->have_domain = true;
+  app()->prim.clear();                 // the set_primitive array no longer fits
+  app()->prim_on_device = false;
   mark_dirty();
   return 0;
 }
@@ -367,6 +380,7 @@ int mara_advance(double *P, size_t n, double dt, double *kzps, int *nerr)
   if (n != zone_doubles(a)) return fail("[Mara] primitive has the wrong size for the domain", MARAGPU_E_SIZE);
   const auto t0 = std::chrono::steady_clock::now();
   int rc = maragpu_upload_prim(a->ctx, P, n);
+  a->prim_on_device = false;
   if (rc) return fail(std::string("[Mara] upload: ") + maragpu_strerror(rc), rc);
   maragpu_stats st;
   const int errors = maragpu_advance(a->ctx, dt, &st);
@@ -397,22 +411,125 @@ double mara_get_timestep(double cfl)
 int mara_set_primitive(const double *P, size_t n)
 {
   if (int rc = need_app()) return rc;
-  if (int rc = ensure_ctx()) return rc;
-  if (n != zone_doubles(app())) return fail("[Mara] primitive has the wrong size for the domain", MARAGPU_E_SIZE);
-  int rc = maragpu_upload_prim(app()->ctx, P, n);
-  if (rc) return fail(std::string("[Mara] upload: ") + maragpu_strerror(rc), rc);
+  App *a = app();
+  if (!a->have_domain) return fail("[Mara] need a domain to run this, use set_domain");
+  if (!P) return fail("[Mara] set_primitive: null buffer");
+  if (n != zone_doubles(a)) return fail("[Mara] primitive has the wrong size for the domain", MARAGPU_E_SIZE);
+  a->prim.assign(P, P + n);
+  a->prim_on_device = false;
   return 0;
 }
 
-int mara_measure_mean_max_divB(double *mean, double *max)
+namespace {
+/* The context's committed state holds the set_primitive array (uploaded when it does not) */
+int measure_ready()
 {
   if (int rc = need_app()) return rc;
-  if (!app()->ctx) {
-    if (mean) *mean = 0.0;
-    if (max) *max = 0.0;
-    return 0;
+  App *a = app();
+  if (int rc = ensure_ctx(false)) return rc;
+  if (a->prim.size() != zone_doubles(a)) return fail("[Mara] no primitive state to measure, use set_primitive");
+  if (!a->prim_on_device) {
+    const int rc = maragpu_upload_prim(a->ctx, a->prim.data(), a->prim.size());
+    if (rc) return fail(std::string("[Mara] upload: ") + maragpu_strerror(rc), rc);
+    a->prim_on_device = true;
   }
-  return maragpu_measure_divB(app()->ctx, mean, max, nullptr);
+  return 0;
+}
+}  // namespace
+
+int mara_measure_mean_max_divB(double *mean, double *max)
+{
+  if (int rc = measure_ready()) return rc;
+  const int rc = maragpu_measure_divB(app()->ctx, mean, max, nullptr);
+  return rc ? fail(std::string("[Mara] measure: ") + maragpu_strerror(rc), rc) : 0;
+}
+
+int mara_set_units(double a, double b, double c)
+{
+  if (int rc = need_app()) return rc;
+  if (!(a > 0.0) || !(b > 0.0) || !(c > 0.0)) return fail("[Mara] set_units: units must be positive");
+  app()->units[0] = a;
+  app()->units[1] = b;
+  app()->units[2] = c;
+  return 0;
+}
+
+namespace {
+int measure_all(maragpu_measures *m)
+{
+  if (int rc = measure_ready()) return rc;
+  const int rc = maragpu_measure(app()->ctx, app()->units, m);
+  return rc ? fail(std::string("[Mara] measure: ") + maragpu_strerror(rc), rc) : 0;
+}
+}  // namespace
+
+int mara_measure_mean_velocity(double out[4])
+{
+  maragpu_measures m;
+  if (int rc = measure_all(&m)) return rc;
+  for (int q = 0; q < 4; ++q) out[q] = m.mean_velocity[q];
+  return 0;
+}
+
+int mara_measure_mean_cons(double out[8])
+{
+  maragpu_measures m;
+  if (int rc = measure_all(&m)) return rc;
+  for (int q = 0; q < 8; ++q) out[q] = m.mean_cons[q];
+  return 0;
+}
+
+int mara_measure_mean_prim(double out[8])
+{
+  maragpu_measures m;
+  if (int rc = measure_all(&m)) return rc;
+  for (int q = 0; q < 8; ++q) out[q] = m.mean_prim[q];
+  return 0;
+}
+
+int mara_measure_mean_energies(double out[4])
+{
+  maragpu_measures m;
+  if (int rc = measure_all(&m)) return rc;
+  for (int q = 0; q < 4; ++q) out[q] = m.mean_energies[q];
+  return 0;
+}
+
+static int measure_pair(double (maragpu_measures::*field)[2], double *a, double *b)
+{
+  maragpu_measures m;
+  if (int rc = measure_all(&m)) return rc;
+  if (a) *a = (m.*field)[0];
+  if (b) *b = (m.*field)[1];
+  return 0;
+}
+
+int mara_measure_mean_max_temperature(double *mean, double *max)
+{
+  return measure_pair(&maragpu_measures::mean_max_temperature, mean, max);
+}
+
+int mara_measure_mean_max_sonic_mach(double *mean, double *max)
+{
+  return measure_pair(&maragpu_measures::mean_max_sonic_mach, mean, max);
+}
+
+int mara_measure_mean_min_alfvenic_mach(double *mean, double *min)
+{
+  return measure_pair(&maragpu_measures::mean_min_alfvenic_mach, mean, min);
+}
+
+int mara_measure_mean_max_magnetic_field(double *mean, double *max)
+{
+  return measure_pair(&maragpu_measures::mean_max_magnetic_field, mean, max);
+}
+
+int mara_measure_max_lorentz_factor(double *max)
+{
+  maragpu_measures m;
+  if (int rc = measure_all(&m)) return rc;
+  if (max) *max = m.max_lorentz_factor;
+  return 0;
 }
 
 struct maragpu_ctx *mara_device_context(void)

@@ -1002,6 +1002,169 @@ __global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, co
 }
 
 /* luaC_mean_max_divB stencil, Mara/mara.cpp:2298-2314, over its loop range */
+/* The Mara.measure table (Mara/mara.cpp:1954-2262) over the interior zones of the committed
+ * state: per zone the quantities of luaC_mean_velocity, _mean_cons, _mean_prim,
+ * _mean_energies, _mean_max_temperature, _mean_max_sonic_mach, _mean_min_alfvenic_mach,
+ * _mean_max_magnetic_field and _max_lorentz_factor for the RMHD fluid and the gamma-law
+ * EOS, in the reference's operation order (no contraction); kMeasSums sums and kMeasExt
+ * extremes per thread over a grid-stride loop, reduced per block in a fixed order into
+ * one row of partials. k_measure_reduce sums the rows, also in a fixed order, so the
+ * result does not change from run to run. */
+constexpr int kMeasSums = 28, kMeasExt = 5, kMeasQ = kMeasSums + kMeasExt, kMeasBlocks = 1024;
+
+struct MeasureUnits { double ErgPerCC, GramsPerCC, mp, kb, K2MeV; };
+
+__device__ __forceinline__ double wave_sum_d(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_measure(const double *__restrict__ P, GridDesc g, double gamma,
+                                                    MeasureUnits mu, double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+  const long T = g.T;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  double sum[kMeasSums];
+#pragma unroll
+  for (int q = 0; q < kMeasSums; ++q) sum[q] = 0.0;
+  double T_max = 0.0, Ms_max = 0.0, Ma_min = 1e20, B_max = 0.0, W_max = 0.0;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ni; t += (long)gridDim.x * blockDim.x) {
+    const long z = interior_zone(g, t);
+    double P0[8], U0[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P0[q] = P[q * T + z];
+    {
+      // Rmhd::PrimToCons (Mara/rmhd.cpp:133-163) with the reference's divisions
+      const double V2 = P0[vx]*P0[vx] + P0[vy]*P0[vy] + P0[vz]*P0[vz];
+      const double B2 = P0[Bx]*P0[Bx] + P0[By]*P0[By] + P0[Bz]*P0[Bz];
+      const double Bv = P0[Bx]*P0[vx] + P0[By]*P0[vy] + P0[Bz]*P0[vz];
+      const double W2 = 1.0 / (1.0 - V2);
+      const double W = sqrt(W2);
+      const double b0 = W*Bv;
+      const double b2 = (B2 + b0*b0) / W2;
+      const double bx = (P0[Bx] + b0 * W*P0[vx]) / W;
+      const double by = (P0[By] + b0 * W*P0[vy]) / W;
+      const double bz = (P0[Bz] + b0 * W*P0[vz]) / W;
+      const double e = (P0[pre] / (gamma - 1)) / P0[rho];
+      const double e_ = e + 0.5 * b2 / P0[rho];
+      const double p_ = P0[pre] + 0.5 * b2;
+      const double h_ = 1 + e_ + p_ / P0[rho];
+      U0[ddd] = P0[rho] * W;
+      U0[tau] = P0[rho] * h_ * W2 - p_ - b0*b0 - U0[ddd];
+      U0[Sx] = P0[rho] * h_ * W2 * P0[vx] - b0*bx;
+      U0[Sy] = P0[rho] * h_ * W2 * P0[vy] - b0*by;
+      U0[Sz] = P0[rho] * h_ * W2 * P0[vz] - b0*bz;
+      U0[Bx] = P0[Bx];
+      U0[By] = P0[By];
+      U0[Bz] = P0[Bz];
+    }
+    const double v2 = P0[vx]*P0[vx] + P0[vy]*P0[vy] + P0[vz]*P0[vz];
+    const double B2 = P0[Bx]*P0[Bx] + P0[By]*P0[By] + P0[Bz]*P0[Bz];
+    const double W0 = 1.0 / sqrt(1.0 - v2);
+    sum[0] += W0;                                   // mean_velocity
+    sum[1] += fabs(P0[vx]);
+    sum[2] += fabs(P0[vy]);
+    sum[3] += fabs(P0[vz]);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      sum[4 + q] += U0[q];                          // mean_cons
+      sum[12 + q] += P0[q];                         // mean_prim
+    }
+    const double T0 = P0[pre];                      // Temperature_p; Internal = T/(Gamma-1)
+    const double u0 = T0 / (gamma - 1);
+    const double rhoh = P0[rho] + u0 + P0[pre];
+    const double e_f = rhoh * W0*W0 - P0[pre] - U0[ddd];
+    sum[20] += U0[tau] - e_f;                       // mean_energies: magnetic
+    sum[21] += P0[rho] * W0 * (W0 - 1);             // kinetic
+    sum[22] += W0 * u0;                             // internal
+    sum[23] += U0[tau];                             // total
+    {
+      // AdiabaticEos::TemperatureMeV, Mara/eos.cpp:37-47
+      const double p = P0[pre] / mu.ErgPerCC, r = P0[rho] / mu.GramsPerCC;
+      const double TM = ((p / r) * (mu.mp / mu.kb)) * mu.K2MeV;
+      sum[24] += TM;
+      if (TM > T_max) T_max = TM;
+    }
+    {
+      const double rh = P0[rho] + T0 / (gamma - 1) + T0;       // SoundSpeed2Sr
+      const double cs2 = gamma * T0 / rh;
+      const double M0 = sqrt((v2 / (1 - v2)) / (cs2 / (1 - cs2)));
+      sum[25] += M0;
+      if (M0 > Ms_max) Ms_max = M0;
+    }
+    {
+      const double va2 = B2 / (B2 + rhoh);
+      const double M0 = sqrt((v2 / (1 - v2)) / (va2 / (1 - va2)));
+      sum[26] += M0;
+      if (M0 < Ma_min) Ma_min = M0;
+    }
+    {
+      const double B0 = sqrt(B2);
+      sum[27] += B0;
+      if (B0 > B_max) B_max = B0;
+    }
+    if (W0 > W_max) W_max = W0;
+  }
+  // fixed-order block reduction into this block's row of partials
+  __shared__ double red[kBlock / 64][kMeasQ];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < kMeasSums; ++q) {
+    const double v = wave_sum_d(sum[q]);
+    if (lane == 0) red[wave][q] = v;
+  }
+  double ext[kMeasExt] = {T_max, Ms_max, Ma_min, B_max, W_max};
+#pragma unroll
+  for (int q = 0; q < kMeasExt; ++q) {
+    double v = ext[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(v, off, 64);
+      v = q == 2 ? (o < v ? o : v) : (o > v ? o : v);
+    }
+    if (lane == 0) red[wave][kMeasSums + q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kMeasQ) {
+    const int q = threadIdx.x;
+    double v = red[0][q];
+    for (int w = 1; w < kBlock / 64; ++w) {
+      const double o = red[w][q];
+      if (q < kMeasSums) v += o;
+      else if (q == kMeasSums + 2) v = o < v ? o : v;
+      else v = o > v ? o : v;
+    }
+    part[(long)blockIdx.x * kMeasQ + q] = v;
+  }
+}
+
+/* One block per quantity: the nb rows of partials in a fixed order */
+__global__ __launch_bounds__(kBlock) void k_measure_reduce(const double *__restrict__ part, int nb,
+                                                           double *__restrict__ out)
+{
+  const int q = blockIdx.x;
+  const bool is_min = q == kMeasSums + 2, is_sum = q < kMeasSums;
+  double v = is_sum ? 0.0 : (is_min ? 1e20 : 0.0);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    const double o = part[(long)b * kMeasQ + q];
+    v = is_sum ? v + o : is_min ? (o < v ? o : v) : (o > v ? o : v);
+  }
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double a = red[threadIdx.x], o = red[threadIdx.x + s];
+      red[threadIdx.x] = is_sum ? a + o : is_min ? (o < a ? o : a) : (o > a ? o : a);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = red[0];
+}
+
 __global__ void k_divB(const double *__restrict__ P, GridDesc g, double *out /* sum, max, absmax */)
 {
 #pragma clang fp contract(off)
@@ -1165,6 +1328,8 @@ struct maragpu_ctx {
   // of the current sweep in the flux precision
   double *Uw = nullptr;
   void *FZ = nullptr;
+  // the measure table: per-block partials and the reduced values
+  double *dmeas = nullptr;
 };
 
 namespace {
@@ -1349,6 +1514,7 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->redo.z);
   hipFree(c->Uw);
   hipFree(c->FZ);
+  hipFree(c->dmeas);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1581,6 +1747,54 @@ int maragpu_measure_divB(maragpu_ctx *c, double *mean, double *max, double *absm
   if (mean) *mean = r[0] / (double)g.T / glb;
   if (max) *max = smax;
   if (absmax) *absmax = r[2] / g.min_dx;
+  return MARAGPU_OK;
+}
+
+int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
+{
+  if (!c || !out) return MARAGPU_E_INVALID;
+  // PhysicalUnits(Length, Mass, Time), Mara/units.cpp:3-6 and :92-141; cgs by default
+  // (HydroModule's units, Mara/hydro.cpp:20); constants of Mara/eos.hpp:12-15
+  const double Length = units ? units[0] : 1.0, Mass = units ? units[1] : 1.0, Time = units ? units[2] : 1.0;
+  if (!(Length > 0.0) || !(Mass > 0.0) || !(Time > 0.0)) return MARAGPU_E_INVALID;
+  const double Gram = 1.0 / Mass, Centimeter = 1.0 / Length, Second = 1.0 / Time;
+  const double Erg = Gram * std::pow(Centimeter / Second, 2.0);
+  const double MeV = 1.60217648740e-06 * Erg;
+  const double Gauss = std::sqrt(Erg / std::pow(Centimeter, 3.0) / (4 * M_PI));
+  MeasureUnits mu;
+  mu.GramsPerCC = Gram / std::pow(Centimeter, 3.0);
+  mu.ErgPerCC = MeV / std::pow(1e-13 * Centimeter, 3) * 1e-33;
+  mu.mp = 1.67262177774e-24 * Gram;
+  mu.kb = 1.38064881300e-16 * Erg;
+  mu.K2MeV = 8.621738e-11;
+  if (!c->dmeas && hipMalloc(&c->dmeas, (size_t)(kMeasBlocks + 1) * kMeasQ * sizeof(double)) != hipSuccess) {
+    c->dmeas = nullptr;
+    return MARAGPU_E_NOMEM;
+  }
+  const GridDesc &g = c->g;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  const int nb = (int)std::min<long>(kMeasBlocks, (ni + kBlock - 1) / kBlock);
+  double *res = c->dmeas + (size_t)kMeasBlocks * kMeasQ;
+  k_measure<<<nb, kBlock, 0, c->stream>>>(c->Pc, g, c->cfg.gamma, mu, c->dmeas);
+  k_measure_reduce<<<kMeasQ, kBlock, 0, c->stream>>>(c->dmeas, nb, res);
+  double r[kMeasQ];
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  const int ttl = g.N[0] * g.N[1] * g.N[2];           // TTL_ZONES, an int in the reference
+  for (int q = 0; q < 4; ++q) out->mean_velocity[q] = r[q] / ttl;
+  for (int q = 0; q < 8; ++q) out->mean_cons[q] = r[4 + q] / ttl;
+  for (int q = 0; q < 8; ++q) out->mean_prim[q] = r[12 + q] / ttl;
+  for (int q = 0; q < 4; ++q) out->mean_energies[q] = r[20 + q] / ttl;
+  out->mean_max_temperature[0] = r[24] / ttl;
+  out->mean_max_temperature[1] = r[kMeasSums + 0];
+  out->mean_max_sonic_mach[0] = r[25] / ttl;
+  out->mean_max_sonic_mach[1] = r[kMeasSums + 1];
+  out->mean_min_alfvenic_mach[0] = r[26] / ttl;
+  out->mean_min_alfvenic_mach[1] = r[kMeasSums + 2];
+  out->mean_max_magnetic_field[0] = (1.0 / Gauss) * r[27] / ttl;
+  out->mean_max_magnetic_field[1] = (1.0 / Gauss) * r[kMeasSums + 3];
+  out->max_lorentz_factor = r[kMeasSums + 4];
   return MARAGPU_OK;
 }
 

@@ -39,11 +39,54 @@ def _chk(rc):
     return rc
 
 
+def _pair(fn):
+    a, b = C.c_double(), C.c_double()
+    _chk(fn(C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def _array(fn, n):
+    out = (C.c_double * n)()
+    _chk(fn(out))
+    return list(out)
+
+
 class _Measure:
+    """Mara.measure (Mara/mara.cpp:174-187): arrays come back as lists, the energies as a
+    dict, the (mean, max|min) pairs as two numbers, as the Lua functions return them."""
+
     def mean_max_divB(self):
-        m, x = C.c_double(), C.c_double()
-        _chk(native.lib().mara_measure_mean_max_divB(C.byref(m), C.byref(x)))
-        return m.value, x.value
+        return _pair(native.lib().mara_measure_mean_max_divB)
+
+    def mean_velocity(self):
+        return _array(native.lib().mara_measure_mean_velocity, 4)
+
+    def mean_cons(self):
+        return _array(native.lib().mara_measure_mean_cons, 8)
+
+    def mean_prim(self):
+        return _array(native.lib().mara_measure_mean_prim, 8)
+
+    def mean_energies(self):
+        e = _array(native.lib().mara_measure_mean_energies, 4)
+        return {"magnetic": e[0], "kinetic": e[1], "internal": e[2], "total": e[3]}
+
+    def mean_max_temperature(self):
+        return _pair(native.lib().mara_measure_mean_max_temperature)
+
+    def mean_max_sonic_mach(self):
+        return _pair(native.lib().mara_measure_mean_max_sonic_mach)
+
+    def mean_min_alfvenic_mach(self):
+        return _pair(native.lib().mara_measure_mean_min_alfvenic_mach)
+
+    def mean_max_magnetic_field(self):
+        return _pair(native.lib().mara_measure_mean_max_magnetic_field)
+
+    def max_lorentz_factor(self):
+        x = C.c_double()
+        _chk(native.lib().mara_measure_max_lorentz_factor(C.byref(x)))
+        return x.value
 
 
 class _Mara:
@@ -57,6 +100,9 @@ class _Mara:
 
     def set_fluid(self, key):
         _chk(native.lib().mara_set_fluid(_s(key)))
+
+    def set_units(self, a, b, c):
+        _chk(native.lib().mara_set_units(float(a), float(b), float(c)))
 
     def set_eos(self, key, gamma=1.4):
         _chk(native.lib().mara_set_eos(_s(key), float(gamma)))

@@ -38,6 +38,10 @@ EXPORTS = [
     "mara_set_riemann", "mara_set_godunov", "mara_set_advance", "mara_config_solver_str",
     "mara_config_solver_num", "mara_config_solver_bool", "mara_set_domain", "mara_advance",
     "mara_get_timestep", "mara_set_primitive", "mara_measure_mean_max_divB", "mara_last_error",
+    "maragpu_measure", "mara_set_units", "mara_measure_mean_velocity", "mara_measure_mean_cons",
+    "mara_measure_mean_prim", "mara_measure_mean_energies", "mara_measure_mean_max_temperature",
+    "mara_measure_mean_max_sonic_mach", "mara_measure_mean_min_alfvenic_mach",
+    "mara_measure_mean_max_magnetic_field", "mara_measure_max_lorentz_factor",
     "mara_device_context", "mara_init_turbulence",
     "maragpu_mlp_create", "maragpu_mlp_destroy", "maragpu_mlp_param_count",
     "maragpu_mlp_serialized_size", "maragpu_mlp_save", "maragpu_mlp_load", "maragpu_mlp_get_params",
@@ -153,6 +157,17 @@ def lib():
         "mara_get_timestep": (C.c_double, [C.c_double]),
         "mara_set_primitive": (C.c_int, [_dp, _sz]),
         "mara_measure_mean_max_divB": (C.c_int, [_dp, _dp]),
+        "maragpu_measure": (C.c_int, [_vp, _dp, C.POINTER(Measures)]),
+        "mara_set_units": (C.c_int, [C.c_double, C.c_double, C.c_double]),
+        "mara_measure_mean_velocity": (C.c_int, [_dp]),
+        "mara_measure_mean_cons": (C.c_int, [_dp]),
+        "mara_measure_mean_prim": (C.c_int, [_dp]),
+        "mara_measure_mean_energies": (C.c_int, [_dp]),
+        "mara_measure_mean_max_temperature": (C.c_int, [_dp, _dp]),
+        "mara_measure_mean_max_sonic_mach": (C.c_int, [_dp, _dp]),
+        "mara_measure_mean_min_alfvenic_mach": (C.c_int, [_dp, _dp]),
+        "mara_measure_mean_max_magnetic_field": (C.c_int, [_dp, _dp]),
+        "mara_measure_max_lorentz_factor": (C.c_int, [_dp]),
         "mara_last_error": (C.c_char_p, []),
         "mara_device_context": (_vp, []),
         "mara_init_turbulence": (C.c_int, [C.POINTER(ICParams), _dp, _sz]),
@@ -235,6 +250,26 @@ def make_config(N=(64, 64, 64), Ng=3, x0=(0.0, 0.0, 0.0), x1=(1.0, 1.0, 1.0), ga
     return c
 
 
+class Measures(C.Structure):
+    """maragpu_measures: the Mara.measure table."""
+    _fields_ = [("mean_velocity", C.c_double * 4), ("mean_cons", C.c_double * 8),
+                ("mean_prim", C.c_double * 8), ("mean_energies", C.c_double * 4),
+                ("mean_max_temperature", C.c_double * 2), ("mean_max_sonic_mach", C.c_double * 2),
+                ("mean_min_alfvenic_mach", C.c_double * 2), ("mean_max_magnetic_field", C.c_double * 2),
+                ("max_lorentz_factor", C.c_double)]
+
+    def as_dict(self):
+        e = list(self.mean_energies)
+        return {"mean_velocity": list(self.mean_velocity), "mean_cons": list(self.mean_cons),
+                "mean_prim": list(self.mean_prim),
+                "mean_energies": {"magnetic": e[0], "kinetic": e[1], "internal": e[2], "total": e[3]},
+                "mean_max_temperature": tuple(self.mean_max_temperature),
+                "mean_max_sonic_mach": tuple(self.mean_max_sonic_mach),
+                "mean_min_alfvenic_mach": tuple(self.mean_min_alfvenic_mach),
+                "mean_max_magnetic_field": tuple(self.mean_max_magnetic_field),
+                "max_lorentz_factor": float(self.max_lorentz_factor)}
+
+
 class Context:
     """A device-resident grid (maragpu_ctx)."""
 
@@ -277,6 +312,13 @@ class Context:
         m, x, a = C.c_double(), C.c_double(), C.c_double()
         check(lib().maragpu_measure_divB(self.h, C.byref(m), C.byref(x), C.byref(a)), "divB")
         return m.value, x.value, a.value
+
+    def measure(self, units=None):
+        """The Mara.measure table of the committed state; units = (Length, Mass, Time)."""
+        m = Measures()
+        u = None if units is None else (C.c_double * 3)(*[float(x) for x in units])
+        check(lib().maragpu_measure(self.h, u, C.byref(m)), "measure")
+        return m.as_dict()
 
     def attach_mlp(self, net: "Mlp"):
         """Use `net` for the learned c2p guess (the context keeps a reference)."""

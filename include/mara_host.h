@@ -22,7 +22,11 @@
  *   mara_advance                     luaC_Mara_advance               (:297-341) -> kz/s, errors
  *   mara_get_timestep                luaC_Mara_get_timestep          (:374-381)
  *   mara_set_primitive               luaC_set_primitive              (:1284-1295)
+ *   mara_set_units                   luaC_set_units                  (:869-883)
  *   mara_measure_mean_max_divB       luaC_mean_max_divB              (:2264-2324)
+ *   mara_measure_mean_velocity, _mean_cons, _mean_prim, _mean_energies, _mean_max_temperature,
+ *   _mean_max_sonic_mach, _mean_min_alfvenic_mach, _mean_max_magnetic_field, _max_lorentz_factor
+ *                                    the Mara.measure table          (:1954-2262)
  *   mara_init_turbulence             (synthetic data; the role of an init_prim callback, :531-632)
  */
 #ifndef MARA_HOST_H
@@ -53,6 +57,22 @@ int mara_advance(double *P, size_t n, double dt, double *kzps, int *nerr);
 double mara_get_timestep(double cfl);
 int mara_set_primitive(const double *P, size_t n);
 int mara_measure_mean_max_divB(double *mean, double *max);
+/* Mara.set_units(a, b, c) builds PhysicalUnits(a, b, c), whose parameters are (Length, Mass,
+ * Time) although the Lua arguments are read as (Mass, Length, Time) (Mara/mara.cpp:871-877,
+ * Mara/units.cpp:3-6): a sets the length unit, b the mass unit. Kept as is. */
+int mara_set_units(double a, double b, double c);
+/* The measures, mean_max_divB included, read the array of the last set_primitive
+ * (Mara->PrimitiveArray, which advance does not change), uploaded to the device on demand;
+ * they need a domain and the fluid, not the rest of the solver. */
+int mara_measure_mean_velocity(double out[4]);       /* <W>, <|vx|>, <|vy|>, <|vz|> */
+int mara_measure_mean_cons(double out[8]);
+int mara_measure_mean_prim(double out[8]);
+int mara_measure_mean_energies(double out[4]);       /* magnetic, kinetic, internal, total */
+int mara_measure_mean_max_temperature(double *mean, double *max);
+int mara_measure_mean_max_sonic_mach(double *mean, double *max);
+int mara_measure_mean_min_alfvenic_mach(double *mean, double *min);
+int mara_measure_mean_max_magnetic_field(double *mean, double *max);
+int mara_measure_max_lorentz_factor(double *max);
 const char *mara_last_error(void);
 /* the device context behind the application (created on first use), for callers that
  * want to keep the state on the device between steps */

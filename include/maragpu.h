@@ -20,6 +20,7 @@
  *                                      fsplit = 'llf'; Mara/weno-split.cpp:13-245)
  *   maragpu_get_timestep               luaC_Mara_get_timestep (Mara/mara.cpp:374-381)
  *   maragpu_measure_divB               luaC_mean_max_divB (Mara/mara.cpp:2264-2324)
+ *   maragpu_measure                    the other Mara.measure functions (Mara/mara.cpp:1954-2262)
  *   maragpu_batch_prim_to_cons         AdiabaticIdealRmhd::PrimToCons (Mara/rmhd.cpp:133-163)
  *   maragpu_batch_cons_to_prim         AdiabaticIdealRmhd::ConsToPrim cascade (Mara/rmhd.cpp:59-132)
  *                                      over the rmhd-c2p.h C API (Mara/rmhd-c2p.h:22-35)
@@ -111,6 +112,21 @@ int maragpu_advance(maragpu_ctx *ctx, double dt, maragpu_stats *stats);
 double maragpu_get_timestep(maragpu_ctx *ctx, double cfl);
 /* mean and signed max in the reference's convention; absmax = max |corner div B| / dx */
 int maragpu_measure_divB(maragpu_ctx *ctx, double *mean, double *max, double *absmax);
+/* The Mara.measure table over the interior zones of the committed state (Mara/mara.cpp:1954-2262):
+ * volume averages (sum / N^3) and extremes. units = {Length, Mass, Time} of PhysicalUnits
+ * (Mara/units.cpp:3-6), NULL for Mara's default cgs (1, 1, 1). */
+typedef struct maragpu_measures {
+  double mean_velocity[4];           /* <W>, <|vx|>, <|vy|>, <|vz|>   luaC_mean_velocity (:1954-1980) */
+  double mean_cons[8];               /* <U>                           luaC_mean_cons (:1982-2013) */
+  double mean_prim[8];               /* <P>                           luaC_mean_prim (:2015-2041) */
+  double mean_energies[4];           /* magnetic, kinetic, internal, total  luaC_mean_energies (:2043-2103) */
+  double mean_max_temperature[2];    /* MeV                           luaC_mean_max_temperature (:2105-2126) */
+  double mean_max_sonic_mach[2];     /*                               luaC_mean_max_sonic_mach (:2128-2165) */
+  double mean_min_alfvenic_mach[2];  /* mean, min                     luaC_mean_min_alfvenic_mach (:2167-2201) */
+  double mean_max_magnetic_field[2]; /* Gauss                         luaC_mean_max_magnetic_field (:2203-2230) */
+  double max_lorentz_factor;         /*                               luaC_max_lorentz_factor (:2232-2258) */
+} maragpu_measures;
+int maragpu_measure(maragpu_ctx *ctx, const double *units, maragpu_measures *out);
 /* device time of the dominant kernel over the last advance (ms, per launch) and launches */
 int maragpu_kernel_times(maragpu_ctx *ctx, double *flux_ms, int *flux_launches, double *c2p_ms,
                          int *c2p_launches, double *update_ms, int *update_launches);

@@ -21,6 +21,7 @@
 #include "counted.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -549,6 +550,113 @@ void oracle_mean_max_divB(void *h, const double *P, double *mean, double *max, d
   if (mean) *mean = sum / T / glb;
   if (max) *max = mx;
   if (absmax) *absmax = amx / g.min_dx;
+}
+
+/* The Mara.measure table on a primitive array (Mara/mara.cpp:1954-2262): sums over the
+ * interior zones in memory order (absolute_index_is_ghost, :2326), divided by the global
+ * interior count, and extremes, as luaC_mean_velocity (:1954), luaC_mean_cons (:1982),
+ * luaC_mean_prim (:2015), luaC_mean_energies (:2043), luaC_mean_max_temperature (:2105),
+ * luaC_mean_max_sonic_mach (:2128), luaC_mean_min_alfvenic_mach (:2167),
+ * luaC_mean_max_magnetic_field (:2203) and luaC_max_lorentz_factor (:2232), for the
+ * AdiabaticIdealRmhd fluid and the gamma-law AdiabaticEos (Mara/eos.cpp:5-47).
+ * units = PhysicalUnits(Length, Mass, Time) (Mara/units.cpp:3-6, :92-141); out holds 33
+ * values in the order of maragpu_measures. */
+void oracle_measures(const int *N, int Ng, const double *P, double gamma, const double *units, double *out)
+{
+  // PhysicalUnits, Mara/units.cpp:92-141, and the constants of Mara/eos.hpp:12-15
+  const double Length = units[0], Mass = units[1], Time = units[2];
+  const double Gram = 1.0 / Mass, Centimeter = 1.0 / Length, Second = 1.0 / Time;
+  const double Erg = Gram * std::pow(Centimeter / Second, 2.0);
+  const double MeV = 1.60217648740e-06 * Erg;
+  const double Gauss = std::sqrt(Erg / std::pow(Centimeter, 3.0) / (4 * M_PI));
+  const double GramsPerCC = Gram / std::pow(Centimeter, 3.0);
+  const double MeVPerFm3 = MeV / std::pow(1e-13 * Centimeter, 3);
+  const double ErgPerCC = MeVPerFm3 * 1e-33;
+  const double mp = 1.67262177774e-24 * Gram;
+  const double kb = 1.38064881300e-16 * Erg;
+  const double Kelvin_to_MeV = 8.621738e-11;          // Mara/eos.cpp:2
+
+  const int n0 = N[0] + 2 * Ng, n1 = N[1] + 2 * Ng, n2 = N[2] + 2 * Ng;
+  double Vrms[4] = {0, 0, 0, 0}, Uavg[8] = {0}, Pavg[8] = {0};
+  double magnetic = 0, kinetic = 0, internal = 0, total = 0;
+  double T_ttl = 0, T_max = 0, Ms_ttl = 0, Ms_max = 0, Ma_ttl = 0, Ma_min = 1e20;
+  double B_ttl = 0, B_max = 0, W_max = 0;
+  for (int i = Ng; i < n0 - Ng; ++i)
+    for (int j = Ng; j < n1 - Ng; ++j)
+      for (int k = Ng; k < n2 - Ng; ++k) {
+        const double *P0 = P + (((long)i * n1 + j) * n2 + k) * 8;
+        double U0[8];
+        prim_to_cons(P0, U0, gamma);
+        const double v2 = P0[vx]*P0[vx] + P0[vy]*P0[vy] + P0[vz]*P0[vz];
+        const double B2 = P0[Bx]*P0[Bx] + P0[By]*P0[By] + P0[Bz]*P0[Bz];
+        const double W0 = 1.0 / std::sqrt(1.0 - v2);
+        // luaC_mean_velocity
+        Vrms[0] += W0;
+        Vrms[1] += std::fabs(P0[vx]);
+        Vrms[2] += std::fabs(P0[vy]);
+        Vrms[3] += std::fabs(P0[vz]);
+        for (int q = 0; q < 8; ++q) { Uavg[q] += U0[q]; Pavg[q] += P0[q]; }
+        // luaC_mean_energies: T = Temperature_p = p, u = Internal = T/(Gamma-1)
+        const double T0 = P0[pre];
+        const double u0 = T0 / (gamma - 1);
+        const double rhoh = P0[rho] + u0 + P0[pre];
+        const double e_f = rhoh * W0*W0 - P0[pre] - U0[ddd];
+        const double e_m = U0[tau] - e_f;
+        magnetic += e_m;
+        kinetic += P0[rho] * W0 * (W0 - 1);
+        internal += W0 * u0;
+        total += U0[tau];
+        // luaC_mean_max_temperature: AdiabaticEos::TemperatureMeV
+        {
+          double p = P0[pre], r = P0[rho];
+          p /= ErgPerCC;
+          r /= GramsPerCC;
+          const double T_Kelvin = (p / r) * (mp / kb);
+          const double TM = T_Kelvin * Kelvin_to_MeV;
+          T_ttl += TM;
+          if (TM > T_max) T_max = TM;
+        }
+        // luaC_mean_max_sonic_mach: SoundSpeed2Sr = Gamma T / rhoh
+        {
+          const double rh = P0[rho] + T0 / (gamma - 1) + T0;
+          const double cs2 = gamma * T0 / rh;
+          const double M0 = std::sqrt((v2 / (1 - v2)) / (cs2 / (1 - cs2)));
+          Ms_ttl += M0;
+          if (M0 > Ms_max) Ms_max = M0;
+        }
+        // luaC_mean_min_alfvenic_mach
+        {
+          const double va2 = B2 / (B2 + rhoh);
+          const double M0 = std::sqrt((v2 / (1 - v2)) / (va2 / (1 - va2)));
+          Ma_ttl += M0;
+          if (M0 < Ma_min) Ma_min = M0;
+        }
+        // luaC_mean_max_magnetic_field
+        {
+          const double B0 = std::sqrt(B2);
+          B_ttl += B0;
+          if (B0 > B_max) B_max = B0;
+        }
+        // luaC_max_lorentz_factor
+        if (W0 > W_max) W_max = W0;
+      }
+  const int ttl = N[0] * N[1] * N[2];                   // TTL_ZONES, an int in the reference
+  for (int q = 0; q < 4; ++q) out[q] = Vrms[q] / ttl;
+  for (int q = 0; q < 8; ++q) out[4 + q] = Uavg[q] / ttl;
+  for (int q = 0; q < 8; ++q) out[12 + q] = Pavg[q] / ttl;
+  out[20] = magnetic / ttl;
+  out[21] = kinetic / ttl;
+  out[22] = internal / ttl;
+  out[23] = total / ttl;
+  out[24] = T_ttl / ttl;
+  out[25] = T_max;
+  out[26] = Ms_ttl / ttl;
+  out[27] = Ms_max;
+  out[28] = Ma_ttl / ttl;
+  out[29] = Ma_min;
+  out[30] = (1.0 / Gauss) * B_ttl / ttl;
+  out[31] = (1.0 / Gauss) * B_max;
+  out[32] = W_max;
 }
 
 /* ---------------------------------------------------------------------------

@@ -105,6 +105,16 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
+def measures_dict(v):
+    """Names the 33 values of the measure table (the layout of maragpu_measures)."""
+    v = [float(x) for x in v]
+    return {"mean_velocity": v[0:4], "mean_cons": v[4:12], "mean_prim": v[12:20],
+            "mean_energies": {"magnetic": v[20], "kinetic": v[21], "internal": v[22], "total": v[23]},
+            "mean_max_temperature": (v[24], v[25]), "mean_max_sonic_mach": (v[26], v[27]),
+            "mean_min_alfvenic_mach": (v[28], v[29]), "mean_max_magnetic_field": (v[30], v[31]),
+            "max_lorentz_factor": v[32]}
+
+
 class Lib:
     """One of the two CPU implementations, loaded through ctypes."""
 
@@ -147,9 +157,20 @@ class Lib:
                                                       C.c_double, _ip, _ip]
             L.oracle_eigensystem_batch.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _ip, C.c_int,
                                                    C.c_double]
+            L.oracle_measures.argtypes = [_ip, C.c_int, _dp, C.c_double, _dp, _dp]
 
     def _f(self, name):
         return getattr(self.lib, self.p + name)
+
+    def measures(self, P, Ng=3, gamma=4.0 / 3.0, units=(1.0, 1.0, 1.0)):
+        """The Mara.measure table (oracle only) of an AoS primitive array with Ng ghosts;
+        units = PhysicalUnits(Length, Mass, Time)."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        N = np.array([s - 2 * Ng for s in P.shape[:3]], dtype=np.int32)
+        u = np.array(units, dtype=np.float64)
+        out = np.zeros(33)
+        self.lib.oracle_measures(_i(N), Ng, _d(P), gamma, _d(u), _d(out))
+        return measures_dict(out)
 
     # -- point functions --------------------------------------------------
     def prim_to_cons(self, P, gamma):

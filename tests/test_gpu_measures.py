@@ -1,0 +1,108 @@
+"""The device's Mara.measure table against the oracle's on the same primitive state.
+
+Per zone both sides evaluate the reference's formulas in the same order without
+contraction, so the extremes agree exactly; the sums are reduced in a different order
+(the device's is fixed, so repeated calls agree bitwise), which leaves ~1e-15 relative
+differences in the means (absolute 1e-15 where a mean cancels to nearly zero)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+G = 4.0 / 3.0
+
+
+def flat(m):
+    e = m["mean_energies"]
+    return np.array(list(m["mean_velocity"]) + list(m["mean_cons"]) + list(m["mean_prim"]) +
+                    [e["magnetic"], e["kinetic"], e["internal"], e["total"]] +
+                    list(m["mean_max_temperature"]) + list(m["mean_max_sonic_mach"]) +
+                    list(m["mean_min_alfvenic_mach"]) + list(m["mean_max_magnetic_field"]) +
+                    [m["max_lorentz_factor"]])
+
+
+EXTREMES = [25, 27, 29, 31, 32]
+
+
+def check(dev, ref):
+    a, b = flat(dev), flat(ref)
+    assert np.array_equal(a[EXTREMES], b[EXTREMES]), (a[EXTREMES], b[EXTREMES])
+    means = np.setdiff1d(np.arange(33), EXTREMES)
+    scale = np.maximum(np.abs(b[means]), 1e-300)
+    # absolute floor: means that cancel to ~1e-17 (the mean of a solenoidal field
+    # perturbation) carry the round-off of summands of order one
+    assert (np.abs(a[means] - b[means]) <= 1e-13 * scale + 1e-15).all(), np.abs(a[means] - b[means]) / scale
+
+
+@pytest.mark.parametrize("units", [None, (3.0, 5.0, 0.5)])
+def test_measures_vs_oracle(gpu, oracle_lib, units):
+    N = 20
+    P0 = gpu.init_turbulence((N, N, N), v_rms=0.5, four_velocity=True, dB_rms=0.4, A_kmax=3.0, seed=12)
+    ctx = gpu.Context(gpu.make_config(N=(N, N, N), extrap="plm", riemann="hlld", advance="rk2"))
+    ctx.upload(P0)
+    check(ctx.measure(units), oracle_lib.measures(P0, gamma=G, units=units or (1.0, 1.0, 1.0)))
+    # after two steps: the committed state
+    dt = 0.0
+    for _ in range(2):
+        e, _ = ctx.advance(dt)
+        assert e == 0
+        dt = ctx.get_timestep(0.4)
+    P = ctx.download()
+    a = ctx.measure(units)
+    check(a, oracle_lib.measures(P, gamma=G, units=units or (1.0, 1.0, 1.0)))
+    assert np.array_equal(flat(a), flat(ctx.measure(units)))     # deterministic
+    ctx.close()
+
+
+def test_ghosts_do_not_count_on_the_device(gpu):
+    N = 10
+    P = gpu.init_turbulence((N, N, N), v_rms=0.3, dB_rms=0.2, seed=9)
+    ctx = gpu.Context(gpu.make_config(N=(N, N, N)))
+    ctx.upload(P)
+    a = flat(ctx.measure())
+    P2 = P.copy()
+    mask = np.ones(P.shape[:3], bool)
+    mask[3:-3, 3:-3, 3:-3] = False
+    P2[mask] = np.nan
+    ctx.upload(P2)
+    assert np.array_equal(a, flat(ctx.measure()))
+    ctx.close()
+
+
+def test_mara_measure_verbs(gpu, oracle_lib):
+    """Mara.set_units and the Mara.measure table through the Lua-facing verbs."""
+    from ctf_amd.mara import Mara, MaraError
+    N = 12
+    Mara.start()
+    with pytest.raises(MaraError, match="need a domain"):
+        Mara.measure.mean_prim()
+    Mara.set_fluid("rmhd")
+    Mara.set_eos("gamma-law", G)
+    Mara.set_domain((0, 0, 0), (1, 1, 1), (N, N, N), 8, 3)
+    P = gpu.init_turbulence((N, N, N), v_rms=0.4, dB_rms=0.3, seed=2)
+    Mara.set_primitive(P)
+    Mara.set_units(2.0, 7.0, 0.25)           # PhysicalUnits(Length=2, Mass=7, Time=0.25)
+    ref = oracle_lib.measures(P, gamma=G, units=(2.0, 7.0, 0.25))
+    dev = {"mean_velocity": Mara.measure.mean_velocity(), "mean_cons": Mara.measure.mean_cons(),
+           "mean_prim": Mara.measure.mean_prim(), "mean_energies": Mara.measure.mean_energies(),
+           "mean_max_temperature": Mara.measure.mean_max_temperature(),
+           "mean_max_sonic_mach": Mara.measure.mean_max_sonic_mach(),
+           "mean_min_alfvenic_mach": Mara.measure.mean_min_alfvenic_mach(),
+           "mean_max_magnetic_field": Mara.measure.mean_max_magnetic_field(),
+           "max_lorentz_factor": Mara.measure.max_lorentz_factor()}
+    check(dev, ref)
+    # advance changes the caller's buffer, not the set_primitive array the measures read
+    Mara.set_godunov("plm-split")
+    Mara.set_riemann("hlld")
+    Mara.set_advance("rk2")
+    Mara.set_boundary("periodic")
+    Q = P.copy()
+    for dt in (0.0, 0.01):
+        _, ne = Mara.advance(Q, dt)
+        assert ne == 0
+    assert not np.array_equal(Q, P)
+    assert Mara.measure.mean_prim() == dev["mean_prim"]
+    Mara.set_primitive(Q)
+    check({**dev, "mean_prim": Mara.measure.mean_prim()},
+          {**ref, "mean_prim": oracle_lib.measures(Q, gamma=G)["mean_prim"]})
+    Mara.close()
