@@ -21,6 +21,7 @@
 #include "counted.hpp"
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -657,6 +658,295 @@ void oracle_measures(const int *N, int Ng, const double *P, double gamma, const 
   out[30] = (1.0 / Gauss) * B_ttl / ttl;
   out[31] = (1.0 / Gauss) * B_max;
   out[32] = W_max;
+}
+
+/* ---------------------------------------------------------------------------
+ * Driving and cooling (Mara/ou-field.cpp, Mara/ornuhl.hpp, Mara/random.cpp,
+ * Mara/driving.cpp, Mara/cooling.cpp), restated; fields travel in the reference's
+ * serialized byte layout
+ * ------------------------------------------------------------------------- */
+}  // extern "C"
+
+namespace ou {
+
+/* RandomNumberStream, Mara/random.cpp:15-57 (Numerical Recipes' Ran). The reference
+ * scales the 64-bit integer by a float literal, so the product is formed in single
+ * precision before it meets (b - a) in double; kept. */
+struct Ran {
+  unsigned long long u, v, w;
+  unsigned long long next()
+  {
+    u = u * 2862933555777941757ULL + 7046029254386353087ULL;
+    v ^= v >> 17; v ^= v << 31; v ^= v >> 8;
+    w = 4294957665U * (w & 0xffffffff) + (w >> 32);
+    unsigned long long x = u ^ (u << 21);
+    x ^= x >> 35; x ^= x << 4;
+    return (x + v) ^ w;
+  }
+  void seed(int n)
+  {
+    v = 4101842887655102017ULL;
+    w = 1;
+    u = (unsigned long long)(long long)n ^ v; next();
+    v = u; next();
+    w = v; next();
+  }
+  double uniform(double a, double b)
+  {
+    const float f = (float)next() * 5.42101086242752217E-20f;
+    return f * (b - a) + a;
+  }
+};
+
+/* OrnsteinUhlenbeckProcess, Mara/ornuhl.hpp:20-68; the same 40-byte layout */
+struct Proc {
+  Ran r;
+  double theta, sigma;
+};
+static_assert(sizeof(Proc) == 40, "layout of the serialized process");
+
+inline double erfinv(double y)
+{
+  double x = 0.5*std::sqrt(M_PI)*(y + M_PI/12.0*std::pow(y, 3));
+  double f, g;
+  do {
+    f = std::erf(x) - y;
+    g = 2.0/std::sqrt(M_PI) * std::exp(-x*x);
+    x -= f/g;
+  } while (std::fabs(f/g) > 1e-12);
+  return x;
+}
+
+inline double normal(Proc &p, double sig2)
+{
+  const double C = p.r.uniform(-1, 1);
+  return std::sqrt(2*sig2) * erfinv(C) + 0.0;
+}
+
+/* Deviate(complex z, dt): the reference draws its two normals as the arguments of one
+ * constructor call; compiled with g++ the imaginary part is drawn first */
+inline std::complex<double> deviate(Proc &p, std::complex<double> z, double dt)
+{
+  const double im = normal(p, dt);
+  const double re = normal(p, dt);
+  const std::complex<double> dW(re, im);
+  return -p.theta*z*dt + p.sigma*dW;
+}
+
+/* StochasticVectorField3d, Mara/ou-field.cpp */
+struct Field3 {
+  int k1 = 0, numk = 0;
+  double P0 = 0, zeta = 0;
+  std::vector<std::complex<double>> Kx, Ky, Kz, Fx, Fy, Fz;
+  std::vector<Proc> procs;
+
+  void build(double P0_, double zeta_, int k1_, int seed)     // :44-74, Stone spectrum
+  {
+    k1 = k1_; numk = (2*k1 + 1)*(2*k1 + 1)*(2*k1 + 1); P0 = P0_; zeta = zeta_;
+    Kx.assign(numk, 0.0); Ky.assign(numk, 0.0); Kz.assign(numk, 0.0);
+    Fx.assign(numk, 0.0); Fy.assign(numk, 0.0); Fz.assign(numk, 0.0);
+    auto spectrum = [](double k) { return std::pow(k, 6) * std::exp(-8*k/(2*M_PI*2.0)); };
+    const int si = (2*k1 + 1)*(2*k1 + 1), sj = 2*k1 + 1;
+    double totpower = 0.0;
+    for (int i = -k1; i <= k1; ++i)
+      for (int j = -k1; j <= k1; ++j)
+        for (int k = -k1; k <= k1; ++k) {
+          const int m = (i + k1)*si + (j + k1)*sj + (k + k1);
+          Kx[m] = 2*M_PI*i; Ky[m] = 2*M_PI*j; Kz[m] = 2*M_PI*k;
+          const double K[3] = {Kx[m].real(), Ky[m].real(), Kz[m].real()};
+          totpower += spectrum(std::sqrt(K[0]*K[0] + K[1]*K[1] + K[2]*K[2]));
+        }
+    procs.resize(numk);
+    for (int m = 0; m < numk; ++m) {
+      const double K[3] = {Kx[m].real(), Ky[m].real(), Kz[m].real()};
+      const double Pk = spectrum(std::sqrt(K[0]*K[0] + K[1]*K[1] + K[2]*K[2]));
+      procs[m].r.seed(seed + m);
+      procs[m].theta = 1.0;
+      procs[m].sigma = std::sqrt(2*P0*Pk/totpower);
+    }
+  }
+  size_t bytes() const { return sizeof(int) + 2*sizeof(double) + 6*numk*sizeof(std::complex<double>) + numk*sizeof(Proc); }
+  void save(char *b) const                                      // :261-278
+  {
+    auto put = [&](const void *p, size_t n) { std::memcpy(b, p, n); b += n; };
+    const size_t cb = numk*sizeof(std::complex<double>);
+    put(&k1, sizeof(int)); put(&P0, sizeof(double)); put(&zeta, sizeof(double));
+    put(Kx.data(), cb); put(Ky.data(), cb); put(Kz.data(), cb);
+    put(Fx.data(), cb); put(Fy.data(), cb); put(Fz.data(), cb);
+    put(procs.data(), numk*sizeof(Proc));
+  }
+  void load(const char *b)                                      // :108-130
+  {
+    auto get = [&](void *p, size_t n) { std::memcpy(p, b, n); b += n; };
+    get(&k1, sizeof(int));
+    numk = (2*k1 + 1)*(2*k1 + 1)*(2*k1 + 1);
+    for (auto *v : {&Kx, &Ky, &Kz, &Fx, &Fy, &Fz}) v->assign(numk, 0.0);
+    procs.resize(numk);
+    const size_t cb = numk*sizeof(std::complex<double>);
+    get(&P0, sizeof(double)); get(&zeta, sizeof(double));
+    get(Kx.data(), cb); get(Ky.data(), cb); get(Kz.data(), cb);
+    get(Fx.data(), cb); get(Fy.data(), cb); get(Fz.data(), cb);
+    get(procs.data(), numk*sizeof(Proc));
+  }
+  void advance(double dt)                                       // :186-210
+  {
+    for (int m = 0; m < numk; ++m) {
+      std::complex<double> dW[3], dV[3];
+      const double K[3] = {Kx[m].real(), Ky[m].real(), Kz[m].real()};
+      const double kdotk = K[0]*K[0] + K[1]*K[1] + K[2]*K[2];
+      if (kdotk < 1e-8) continue;
+      dW[0] = deviate(procs[m], Fx[m], dt);
+      dW[1] = deviate(procs[m], Fy[m], dt);
+      dW[2] = deviate(procs[m], Fz[m], dt);
+      for (int p = 0; p < 3; ++p) {
+        dV[p] = 0.0;
+        for (int q = 0; q < 3; ++q) {
+          const double Pp = (1 - 2*zeta)*K[p]*K[q]/kdotk + zeta*(p == q);
+          dV[p] += Pp * dW[q] / std::sqrt(1 - 1.25*zeta + 3*zeta*zeta);
+        }
+      }
+      Fx[m] += dV[0]; Fy[m] += dV[1]; Fz[m] += dV[2];
+    }
+  }
+  /* :226-239; the sum of a valarray expression (libstdc++ _Expr::sum) runs from the last
+   * element to the first */
+  void sample(double x, double y, double z, double *F) const
+  {
+    const std::complex<double> X(0, x), Y(0, y), Z(0, z);
+    std::complex<double> s[3];
+    for (int m = numk - 1; m >= 0; --m) {
+      const std::complex<double> e = std::exp(Kx[m]*X + Ky[m]*Y + Kz[m]*Z);
+      const std::complex<double> t[3] = {Fx[m]*e, Fy[m]*e, Fz[m]*e};
+      for (int d = 0; d < 3; ++d) s[d] = m == numk - 1 ? t[d] : s[d] + t[d];
+    }
+    for (int d = 0; d < 3; ++d) F[d] = s[d].real();
+  }
+};
+
+}  // namespace ou
+
+extern "C" {
+
+size_t oracle_ou_new(double P0, double zeta, int k1, int seed, char *buf, size_t cap)
+{
+  ou::Field3 f;
+  f.build(P0, zeta, k1, seed);
+  if (buf && f.bytes() <= cap) f.save(buf);
+  return f.bytes();
+}
+
+void oracle_ou_advance(char *buf, size_t n, double dt, int times)
+{
+  ou::Field3 f;
+  f.load(buf);
+  if (f.bytes() != n) return;
+  for (int t = 0; t < times; ++t) f.advance(dt);
+  f.save(buf);
+}
+
+void oracle_ou_sample(const char *buf, size_t n, long npts, const double *xyz, double *F)
+{
+  ou::Field3 f;
+  f.load(buf);
+  if (f.bytes() != n) return;
+  for (long i = 0; i < npts; ++i) f.sample(xyz[3*i], xyz[3*i + 1], xyz[3*i + 2], F + 3*i);
+}
+
+/* DrivingProcedure on an AoS primitive array with Ng ghosts over [x0, x1]: the field sampled
+ * at every zone centre, ghosts included (ResampleField, Mara/driving.cpp:44-85, x_at of
+ * Mara/simple-cart.hpp:42-44), then Drive_rmhd (:154-186); power = AveragePowerInFields */
+void oracle_drive(const int *N, int Ng, const double *x0, const double *x1, const char *buf, size_t n,
+                  double *P, double dt, double *power)
+{
+  ou::Field3 f;
+  f.load(buf);
+  if (f.bytes() != n) return;
+  const int n0 = N[0] + 2*Ng, n1 = N[1] + 2*Ng, n2 = N[2] + 2*Ng;
+  double dx[3];
+  for (int d = 0; d < 3; ++d) dx[d] = (x1[d] - x0[d]) / N[d];
+  const long T = (long)n0 * n1 * n2;
+  std::vector<double> Fx(T), Fy(T), Fz(T);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n0; ++i)
+    for (int j = 0; j < n1; ++j)
+      for (int k = 0; k < n2; ++k) {
+        const long z = ((long)i*n1 + j)*n2 + k;
+        double F[3];
+        f.sample(x0[0] + dx[0]*(i - Ng + 0.5), x0[1] + dx[1]*(j - Ng + 0.5), x0[2] + dx[2]*(k - Ng + 0.5), F);
+        Fx[z] = F[0]; Fy[z] = F[1]; Fz[z] = F[2];
+      }
+  for (long z = 0; z < T; ++z) {
+    double *P0 = P + 8*z;
+    double uf[4] = {0.0, P0[vx], P0[vy], P0[vz]};
+    const double v2 = uf[1]*uf[1] + uf[2]*uf[2] + uf[3]*uf[3];
+    const double W0 = 1.0 / std::sqrt(1.0 - v2);
+    uf[1] *= W0; uf[1] += Fx[z] * (dt/W0);
+    uf[2] *= W0; uf[2] += Fy[z] * (dt/W0);
+    uf[3] *= W0; uf[3] += Fz[z] * (dt/W0);
+    const double u2 = uf[1]*uf[1] + uf[2]*uf[2] + uf[3]*uf[3];
+    const double W1 = std::sqrt(1.0 + u2);
+    P0[vx] = uf[1]/W1;
+    P0[vy] = uf[2]/W1;
+    P0[vz] = uf[3]/W1;
+    P0[rho] *= W0/W1;
+  }
+  if (power) {
+    // (Fx*Fx + Fy*Fy + Fz*Fz).sum(): a valarray expression, summed from the last zone
+    double s = 0.0;
+    for (long z = T - 1; z >= 0; --z) {
+      const double e = Fx[z]*Fx[z] + Fy[z]*Fy[z] + Fz[z]*Fz[z];
+      s = z == T - 1 ? e : s + e;
+    }
+    *power = s / T;
+  }
+}
+
+/* CoolingModuleT4 (kind 1) or E4 (kind 2), Mara/cooling.cpp:13-103, over every zone of an AoS
+ * primitive array (ghosts included, as there), with the gamma-law EOS in PhysicalUnits
+ * units = {Length, Mass, Time}; removed = the energy taken out, summed in zone order */
+void oracle_cool(const int *N, int Ng, const double *x0, const double *x1, double gamma, const double *units,
+                 int kind, double ref, double t0, double *P, double dt, double *removed)
+{
+  const double Length = units[0], Mass = units[1], Time = units[2];
+  const double Gram = 1.0 / Mass, Centimeter = 1.0 / Length, Second = 1.0 / Time;
+  const double Erg = Gram * std::pow(Centimeter / Second, 2.0);
+  const double MeV = 1.60217648740e-06 * Erg;
+  const double GramsPerCC = Gram / std::pow(Centimeter, 3.0);
+  const double ErgPerCC = MeV / std::pow(1e-13 * Centimeter, 3) * 1e-33;
+  const double mp = 1.67262177774e-24 * Gram, kb = 1.38064881300e-16 * Erg;
+  const double Kelvin_to_MeV = 8.621738e-11;
+  const long T = (long)(N[0] + 2*Ng) * (N[1] + 2*Ng) * (N[2] + 2*Ng);
+  double dxs[3];
+  for (int d = 0; d < 3; ++d) dxs[d] = (x1[d] - x0[d]) / N[d];
+  const double dV = dxs[0]*dxs[1]*dxs[2];
+  double energy = 0.0;
+  for (long z = 0; z < T; ++z) {
+    double *P0 = P + 8*z;
+    double U0[8], U1[8];
+    double T1;
+    if (kind == 1) {
+      double p = P0[pre], r = P0[rho];                 // TemperatureMeV, Mara/eos.cpp:37-47
+      p /= ErgPerCC;
+      r /= GramsPerCC;
+      const double T0 = (p/r) * (mp/kb) * Kelvin_to_MeV;
+      const double T1_MeV = T0 - dt * (ref/t0) * std::pow(T0/ref, 4);
+      double rr = P0[rho];                             // TemperatureArb, :48-58
+      rr /= GramsPerCC;
+      const double T_Kelvin = T1_MeV / Kelvin_to_MeV;
+      T1 = ((kb*T_Kelvin) * (rr/mp)) * ErgPerCC;
+    }
+    else {
+      const double T0 = P0[pre];                       // Temperature_p, Internal, Temperature_u
+      const double e0 = (T0 / (gamma - 1)) / P0[rho];
+      const double e1 = e0 - dt * (ref/t0) * std::pow(e0/ref, 4);
+      T1 = (P0[rho]*e1) * (gamma - 1);
+    }
+    prim_to_cons(P0, U0, gamma);
+    P0[pre] = T1;                                      // Pressure(rho, T) = T
+    prim_to_cons(P0, U1, gamma);
+    energy += (U0[tau] - U1[tau])*dV;
+  }
+  if (removed) *removed = energy;
 }
 
 /* ---------------------------------------------------------------------------

@@ -162,6 +162,71 @@ class Lib:
     def _f(self, name):
         return getattr(self.lib, self.p + name)
 
+    # -- driving field and cooling ----------------------------------------
+    def ou_new(self, P0, zeta, k1, seed):
+        """A new StochasticVectorField3d (Stone spectrum) as the reference's serialized bytes."""
+        f = self._f("ou_new")
+        f.restype = C.c_size_t
+        f.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_char_p, C.c_size_t]
+        n = f(P0, zeta, k1, seed, None, 0)
+        buf = C.create_string_buffer(n)
+        f(P0, zeta, k1, seed, buf, n)
+        return buf.raw
+
+    def ou_advance(self, field: bytes, dt, times=1):
+        f = self._f("ou_advance")
+        f.argtypes = [C.c_char_p, C.c_size_t, C.c_double, C.c_int]
+        buf = C.create_string_buffer(field, len(field))
+        f(buf, len(field), dt, times)
+        return buf.raw
+
+    def ou_sample(self, field: bytes, xyz):
+        f = self._f("ou_sample")
+        f.argtypes = [C.c_char_p, C.c_size_t, C.c_long, _dp, _dp]
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        F = np.zeros_like(xyz)
+        f(field, len(field), len(xyz), _d(xyz), _d(F))
+        return F
+
+    def drive(self, solver: "Solver", field: bytes, P, dt):
+        """Drive (DrivingProcedure) P in place on the solver's domain; returns the mean power."""
+        assert P.dtype == np.float64 and P.flags.c_contiguous
+        power = C.c_double()
+        if self.which == "oracle":
+            s = solver.setup
+            N = np.array(s.N, dtype=np.int32)
+            x0 = np.array(s.x0, dtype=np.float64)
+            x1 = np.array(s.x1, dtype=np.float64)
+            self.lib.oracle_drive.argtypes = [_ip, C.c_int, _dp, _dp, C.c_char_p, C.c_size_t, _dp,
+                                              C.c_double, C.POINTER(C.c_double)]
+            self.lib.oracle_drive(_i(N), s.Ng, _d(x0), _d(x1), field, len(field), _d(P), dt, C.byref(power))
+        else:
+            self.lib.ref_drive.argtypes = [C.c_char_p, C.c_size_t, _dp, C.c_double, C.POINTER(C.c_double)]
+            self.lib.ref_drive(field, len(field), _d(P), dt, C.byref(power))
+        return power.value
+
+    def cool(self, solver: "Solver", kind, ref, t0, P, dt, units=(1.0, 1.0, 1.0)):
+        """CoolingModuleT4 ('T4') or E4 ('E4') on P in place; returns the energy removed."""
+        assert P.dtype == np.float64 and P.flags.c_contiguous
+        k = {"T4": 1, "E4": 2}[kind]
+        removed = C.c_double()
+        if self.which == "oracle":
+            s = solver.setup
+            N = np.array(s.N, dtype=np.int32)
+            x0 = np.array(s.x0, dtype=np.float64)
+            x1 = np.array(s.x1, dtype=np.float64)
+            u = np.array(units, dtype=np.float64)
+            self.lib.oracle_cool.argtypes = [_ip, C.c_int, _dp, _dp, C.c_double, _dp, C.c_int, C.c_double,
+                                             C.c_double, _dp, C.c_double, C.POINTER(C.c_double)]
+            self.lib.oracle_cool(_i(N), s.Ng, _d(x0), _d(x1), s.gamma, _d(u), k, ref, t0, _d(P), dt,
+                                 C.byref(removed))
+        else:
+            assert tuple(units) == (1.0, 1.0, 1.0)
+            self.lib.ref_cool.argtypes = [C.c_int, C.c_double, C.c_double, _dp, C.c_double,
+                                          C.POINTER(C.c_double)]
+            self.lib.ref_cool(k, ref, t0, _d(P), dt, C.byref(removed))
+        return removed.value
+
     def measures(self, P, Ng=3, gamma=4.0 / 3.0, units=(1.0, 1.0, 1.0)):
         """The Mara.measure table (oracle only) of an AoS primitive array with Ng ghosts;
         units = PhysicalUnits(Length, Mass, Time)."""

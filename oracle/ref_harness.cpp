@@ -10,7 +10,10 @@
 #include "mara.hpp"
 #include "mara_oracle.hpp"   // rmhd_eigensystem: the RMHD eigensystem the reference lacks
 
+#include <cmath>
 #include <cstring>
+#include <sstream>
+#include <string>
 #include <valarray>
 
 extern "C" {
@@ -247,6 +250,84 @@ void ref_riemann_batch(long n, int solver, const double *PL, const double *PR, d
     if (hlld_ok) hlld_ok[z] = HlldRmhdRiemannSolver::NumberOfSuccess - s0;
   }
   delete rs;
+}
+
+/* ---------------------------------------------------------------------------
+ * Driving and cooling: the reference's StochasticVectorField3d (Mara/ou-field.cpp),
+ * DrivingProcedure (Mara/driving.cpp) and CoolingModuleT4/E4 (Mara/cooling.cpp) on the
+ * domain and fluid of the last ref_create. Fields travel as the reference's own
+ * serialized bytes.
+ * ------------------------------------------------------------------------- */
+}  // extern "C"
+
+// the power spectrum luaC_new_ou_field passes (Mara/mara.cpp:1373-1377): k^6 e^(-8k/(2 pi k_peak)), k_peak = 2
+static double harness_stone_spectrum(double k)
+{
+  const double k_peak = 2.0;
+  return std::pow(k, 6) * std::exp(-8 * k / (2 * M_PI * k_peak));
+}
+
+static size_t put_bytes(const std::string &d, char *buf, size_t cap)
+{
+  if (buf && d.size() <= cap) std::memcpy(buf, d.data(), d.size());
+  return d.size();
+}
+
+extern "C" {
+
+size_t ref_ou_new(double P0, double zeta, int k1, int seed, char *buf, size_t cap)
+{
+  StochasticVectorField3d f(P0, zeta, k1, seed, harness_stone_spectrum);
+  std::stringstream out;
+  f.Serialize(out);
+  return put_bytes(out.str(), buf, cap);
+}
+
+void ref_ou_advance(char *buf, size_t n, double dt, int times)
+{
+  std::stringstream in(std::string(buf, n));
+  StochasticVectorField3d f(in);
+  for (int t = 0; t < times; ++t) f.AdvanceField(dt);
+  std::stringstream out;
+  f.Serialize(out);
+  put_bytes(out.str(), buf, n);
+}
+
+void ref_ou_sample(const char *buf, size_t n, long npts, const double *xyz, double *F)
+{
+  std::stringstream in(std::string(buf, n));
+  StochasticVectorField3d f(in);
+  for (long i = 0; i < npts; ++i) {
+    const std::vector<double> v = f.SampleField(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+    for (int d = 0; d < 3; ++d) F[3 * i + d] = v[d];
+  }
+}
+
+/* DrivingProcedure built on the field (it samples every zone of the domain), then Drive on
+ * the AoS primitives P; power = AveragePowerInFields */
+void ref_drive(const char *buf, size_t n, double *P, double dt, double *power)
+{
+  std::stringstream in(std::string(buf, n));
+  DrivingProcedure d(new StochasticVectorField3d(in));
+  const size_t N = App->domain->GetNumberOfZones() * App->domain->get_Nq();
+  std::valarray<double> V(P, N);
+  d.Drive(V, dt);
+  std::memcpy(P, &V[0], N * sizeof(double));
+  if (power) *power = d.AveragePowerInFields();
+}
+
+/* CoolingModuleT4 (kind 1) or CoolingModuleE4 (kind 2) on the AoS primitives P; removed =
+ * EnergyRemoved after the call */
+void ref_cool(int kind, double ref, double t0, double *P, double dt, double *removed)
+{
+  CoolingModule *c = kind == 1 ? (CoolingModule *)new CoolingModuleT4(ref, t0)
+                               : (CoolingModule *)new CoolingModuleE4(ref, t0);
+  const size_t N = App->domain->GetNumberOfZones() * App->domain->get_Nq();
+  std::valarray<double> V(P, N);
+  c->Cool(V, dt);
+  std::memcpy(P, &V[0], N * sizeof(double));
+  if (removed) *removed = c->EnergyRemoved();
+  delete c;
 }
 
 }  // extern "C"
