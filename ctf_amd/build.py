@@ -16,8 +16,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmaragpu.so")
 
-SOURCES = ["maragpu.hip", "mlp.hip", "mara_app.cpp", "initial_data.cpp"]
-HEADERS = ["rmhd_device.hpp", "mlp_device.hpp", "eigensystem_device.hpp"]
+SOURCES = ["maragpu.hip", "mlp.hip", "mara_app.cpp", "initial_data.cpp", "ou_field.cpp"]
+HEADERS = ["rmhd_device.hpp", "mlp_device.hpp", "eigensystem_device.hpp", "ou_field.hpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",

@@ -37,6 +37,14 @@ struct App {
   // (advance does not change it, Mara/mara.cpp:297-341)
   std::vector<double> prim;
   bool prim_on_device = false;         // the context's committed state holds `prim`
+  // driving: the field (advanced by driving_advance) and its bytes when last sampled
+  // (set_driving, driving_resample), which is what the context drives with
+  maragpu_ou_field *field = nullptr;
+  std::vector<unsigned char> sampled;
+  // cooling: 0 none, 1 T4, 2 E4, and the energy removed by contexts since destroyed
+  int cool_kind = 0;
+  double cool_ref = 0.0, cool_t0 = 0.0;
+  double removed_pending = 0.0;
   std::string err;
 
   App()
@@ -122,12 +130,21 @@ int ensure_ctx(bool for_advance = true)
   }
   if (!a->dirty && a->ctx) return 0;
   if (a->ctx) {
+    double e = 0.0;
+    if (maragpu_cooling_energy_removed(a->ctx, &e) == MARAGPU_OK) a->removed_pending += e;
     maragpu_destroy(a->ctx);
     a->ctx = nullptr;
     a->prim_on_device = false;
   }
   int rc = maragpu_create(&a->cfg, &a->ctx);
   if (rc) return fail(std::string("[Mara] device context: ") + maragpu_strerror(rc), rc);
+  if (!a->sampled.empty()) {
+    maragpu_ou_field *f = nullptr;
+    rc = maragpu_ou_field_load(a->sampled.data(), a->sampled.size(), &f);
+    if (!rc) rc = maragpu_set_driving(a->ctx, f);
+    maragpu_ou_field_destroy(f);
+    if (rc) return fail(std::string("[Mara] driving: ") + maragpu_strerror(rc), rc);
+  }
   if (a->cfg.c2p == MARAGPU_C2P_LEARNED) {
     if (!a->mlp && (rc = load_mlp(a))) return rc;
     maragpu_attach_mlp(a->ctx, a->mlp);
@@ -162,6 +179,7 @@ int mara_close(void)
   if (g_app) {
     if (g_app->ctx) maragpu_destroy(g_app->ctx);
     if (g_app->mlp) maragpu_mlp_destroy(g_app->mlp);
+    if (g_app->field) maragpu_ou_field_destroy(g_app->field);
     delete g_app;
     g_app = nullptr;
   }
@@ -379,7 +397,9 @@ int mara_advance(double *P, size_t n, double dt, double *kzps, int *nerr)
   if (!P) return fail("[Mara] advance: null buffer");
   if (n != zone_doubles(a)) return fail("[Mara] primitive has the wrong size for the domain", MARAGPU_E_SIZE);
   const auto t0 = std::chrono::steady_clock::now();
-  int rc = maragpu_upload_prim(a->ctx, P, n);
+  int rc = maragpu_set_cooling(a->ctx, a->cool_kind, a->cool_ref, a->cool_t0, a->units);
+  if (rc) return fail(std::string("[Mara] cooling: ") + maragpu_strerror(rc), rc);
+  rc = maragpu_upload_prim(a->ctx, P, n);
   a->prim_on_device = false;
   if (rc) return fail(std::string("[Mara] upload: ") + maragpu_strerror(rc), rc);
   maragpu_stats st;
@@ -529,6 +549,110 @@ int mara_measure_max_lorentz_factor(double *max)
   maragpu_measures m;
   if (int rc = measure_all(&m)) return rc;
   if (max) *max = m.max_lorentz_factor;
+  return 0;
+}
+
+int mara_new_ou_field(int Nd, double P0, double zeta, int k1, int seed, void *buf, size_t cap, size_t *len)
+{
+  if (int rc = need_app()) return rc;
+  if (Nd != 3) return fail("[Mara] new_ou_field: this build drives 3-d domains only");
+  maragpu_ou_field *f = nullptr;
+  int rc = maragpu_ou_field_create(P0, zeta, k1, seed, &f);
+  if (rc) return fail("[Mara] new_ou_field: invalid parameters", rc);
+  const size_t n = maragpu_ou_field_bytes(f);
+  if (len) *len = n;
+  if (buf && cap >= n) rc = maragpu_ou_field_save(f, buf, cap);
+  maragpu_ou_field_destroy(f);
+  return rc ? fail("[Mara] new_ou_field: " + std::string(maragpu_strerror(rc)), rc) : 0;
+}
+
+int mara_set_driving(const void *bytes, size_t n)
+{
+  if (int rc = need_app()) return rc;
+  App *a = app();
+  if (!a->have_domain) return fail("[Mara] need a domain to run this, use set_domain");
+  maragpu_ou_field *f = nullptr;
+  int rc = maragpu_ou_field_load(bytes, n, &f);
+  if (rc) return fail("[Mara] set_driving: not a serialized 3-d driving field", rc);
+  maragpu_ou_field_destroy(a->field);
+  a->field = f;
+  a->sampled.assign(static_cast<const unsigned char *>(bytes), static_cast<const unsigned char *>(bytes) + n);
+  if (a->ctx && (rc = maragpu_set_driving(a->ctx, f))) return fail(std::string("[Mara] driving: ") + maragpu_strerror(rc), rc);
+  return 0;
+}
+
+int mara_driving_advance(double dt)
+{
+  if (int rc = need_app()) return rc;
+  if (app()->field) maragpu_ou_field_advance(app()->field, dt);
+  return 0;
+}
+
+int mara_driving_resample(void)
+{
+  if (int rc = need_app()) return rc;
+  App *a = app();
+  if (!a->field) return 0;
+  a->sampled.resize(maragpu_ou_field_bytes(a->field));
+  maragpu_ou_field_save(a->field, a->sampled.data(), a->sampled.size());
+  if (a->ctx) {
+    const int rc = maragpu_set_driving(a->ctx, a->field);
+    if (rc) return fail(std::string("[Mara] driving: ") + maragpu_strerror(rc), rc);
+  }
+  return 0;
+}
+
+int mara_driving_serialize(void *buf, size_t cap, size_t *len)
+{
+  if (int rc = need_app()) return rc;
+  App *a = app();
+  const size_t n = a->field ? maragpu_ou_field_bytes(a->field) : 0;
+  if (len) *len = n;
+  if (a->field && buf && cap >= n) maragpu_ou_field_save(a->field, buf, cap);
+  return 0;
+}
+
+int mara_driving_power(double *power)
+{
+  if (int rc = need_app()) return rc;
+  App *a = app();
+  if (!power) return fail("[Mara] driving power: null output");
+  *power = 0.0;
+  if (!a->field) return 0;
+  if (int rc = ensure_ctx(false)) return rc;
+  const int rc = maragpu_driving_power(a->ctx, power);
+  return rc ? fail(std::string("[Mara] driving power: ") + maragpu_strerror(rc), rc) : 0;
+}
+
+int mara_set_cooling(const char *key, double ref, double t0)
+{
+  if (int rc = need_app()) return rc;
+  if (!key) return fail("[Mara] set_cooling: null key");
+  App *a = app();
+  // an unknown key leaves the cooling as it was (Mara/mara.cpp:1198-1217)
+  const int kind = !std::strcmp(key, "T4") ? 1 : !std::strcmp(key, "E4") ? 2 : 0;
+  if (!kind) return 0;
+  if (!(ref > 0.0) || !(t0 > 0.0)) return fail("[Mara] set_cooling: reference value and time must be positive");
+  a->cool_kind = kind;
+  a->cool_ref = ref;
+  a->cool_t0 = t0;
+  return 0;
+}
+
+int mara_cooling_rate(double dt, double *rate)
+{
+  if (int rc = need_app()) return rc;
+  App *a = app();
+  if (!rate) return fail("[Mara] cooling_rate: null output");
+  double e = a->removed_pending;
+  a->removed_pending = 0.0;
+  if (a->ctx) {
+    double d = 0.0;
+    const int rc = maragpu_cooling_energy_removed(a->ctx, &d);
+    if (rc) return fail(std::string("[Mara] cooling_rate: ") + maragpu_strerror(rc), rc);
+    e += d;
+  }
+  *rate = a->cool_kind ? e / dt : 0.0;
   return 0;
 }
 

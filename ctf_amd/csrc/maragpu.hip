@@ -34,6 +34,7 @@
 #include "rmhd_device.hpp"
 #include "mlp_device.hpp"
 #include "eigensystem_device.hpp"
+#include "ou_field.hpp"
 #include "../../include/maragpu.h"
 
 using namespace maragpu;
@@ -1011,14 +1012,65 @@ __global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, co
  * one row of partials. k_measure_reduce sums the rows, also in a fixed order, so the
  * result does not change from run to run. */
 constexpr int kMeasSums = 28, kMeasExt = 5, kMeasQ = kMeasSums + kMeasExt, kMeasBlocks = 1024;
+constexpr int kCoolBlocks = 1024;      // grid of k_cool and k_power (grid-stride), rows of their partial sums
+constexpr int kOuMaxK1 = 6;            // (2 k1 + 1)^3 modes x 48 B of amplitudes in LDS
 
 struct MeasureUnits { double ErgPerCC, GramsPerCC, mp, kb, K2MeV; };
+
+/* PhysicalUnits(Length, Mass, Time) (Mara/units.cpp:3-6, :92-141) and the constants of
+ * Mara/eos.hpp:12-15, as TemperatureMeV / TemperatureArb use them */
+MeasureUnits measure_units(const double *units)
+{
+  const double Length = units[0], Mass = units[1], Time = units[2];
+  const double Gram = 1.0 / Mass, Centimeter = 1.0 / Length, Second = 1.0 / Time;
+  const double Erg = Gram * std::pow(Centimeter / Second, 2.0);
+  const double MeV = 1.60217648740e-06 * Erg;
+  MeasureUnits mu;
+  mu.GramsPerCC = Gram / std::pow(Centimeter, 3.0);
+  mu.ErgPerCC = MeV / std::pow(1e-13 * Centimeter, 3) * 1e-33;
+  mu.mp = 1.67262177774e-24 * Gram;
+  mu.kb = 1.38064881300e-16 * Erg;
+  mu.K2MeV = 8.621738e-11;
+  return mu;
+}
+
+
 
 __device__ __forceinline__ double wave_sum_d(double v)
 {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+/* Rmhd::PrimToCons (Mara/rmhd.cpp:133-163) with the reference's divisions and operation
+ * order (no contraction), for the diagnostics and source steps that must round as the
+ * reference does */
+__device__ __forceinline__ void prim_to_cons_ref(const double *P0, double *U0, double gamma)
+{
+#pragma clang fp contract(off)
+  const double V2 = P0[vx]*P0[vx] + P0[vy]*P0[vy] + P0[vz]*P0[vz];
+  const double B2 = P0[Bx]*P0[Bx] + P0[By]*P0[By] + P0[Bz]*P0[Bz];
+  const double Bv = P0[Bx]*P0[vx] + P0[By]*P0[vy] + P0[Bz]*P0[vz];
+  const double W2 = 1.0 / (1.0 - V2);
+  const double W = sqrt(W2);
+  const double b0 = W*Bv;
+  const double b2 = (B2 + b0*b0) / W2;
+  const double bx = (P0[Bx] + b0 * W*P0[vx]) / W;
+  const double by = (P0[By] + b0 * W*P0[vy]) / W;
+  const double bz = (P0[Bz] + b0 * W*P0[vz]) / W;
+  const double e = (P0[pre] / (gamma - 1)) / P0[rho];
+  const double e_ = e + 0.5 * b2 / P0[rho];
+  const double p_ = P0[pre] + 0.5 * b2;
+  const double h_ = 1 + e_ + p_ / P0[rho];
+  U0[ddd] = P0[rho] * W;
+  U0[tau] = P0[rho] * h_ * W2 - p_ - b0*b0 - U0[ddd];
+  U0[Sx] = P0[rho] * h_ * W2 * P0[vx] - b0*bx;
+  U0[Sy] = P0[rho] * h_ * W2 * P0[vy] - b0*by;
+  U0[Sz] = P0[rho] * h_ * W2 * P0[vz] - b0*bz;
+  U0[Bx] = P0[Bx];
+  U0[By] = P0[By];
+  U0[Bz] = P0[Bz];
 }
 
 __global__ __launch_bounds__(kBlock) void k_measure(const double *__restrict__ P, GridDesc g, double gamma,
@@ -1036,31 +1088,7 @@ __global__ __launch_bounds__(kBlock) void k_measure(const double *__restrict__ P
     double P0[8], U0[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) P0[q] = P[q * T + z];
-    {
-      // Rmhd::PrimToCons (Mara/rmhd.cpp:133-163) with the reference's divisions
-      const double V2 = P0[vx]*P0[vx] + P0[vy]*P0[vy] + P0[vz]*P0[vz];
-      const double B2 = P0[Bx]*P0[Bx] + P0[By]*P0[By] + P0[Bz]*P0[Bz];
-      const double Bv = P0[Bx]*P0[vx] + P0[By]*P0[vy] + P0[Bz]*P0[vz];
-      const double W2 = 1.0 / (1.0 - V2);
-      const double W = sqrt(W2);
-      const double b0 = W*Bv;
-      const double b2 = (B2 + b0*b0) / W2;
-      const double bx = (P0[Bx] + b0 * W*P0[vx]) / W;
-      const double by = (P0[By] + b0 * W*P0[vy]) / W;
-      const double bz = (P0[Bz] + b0 * W*P0[vz]) / W;
-      const double e = (P0[pre] / (gamma - 1)) / P0[rho];
-      const double e_ = e + 0.5 * b2 / P0[rho];
-      const double p_ = P0[pre] + 0.5 * b2;
-      const double h_ = 1 + e_ + p_ / P0[rho];
-      U0[ddd] = P0[rho] * W;
-      U0[tau] = P0[rho] * h_ * W2 - p_ - b0*b0 - U0[ddd];
-      U0[Sx] = P0[rho] * h_ * W2 * P0[vx] - b0*bx;
-      U0[Sy] = P0[rho] * h_ * W2 * P0[vy] - b0*by;
-      U0[Sz] = P0[rho] * h_ * W2 * P0[vz] - b0*bz;
-      U0[Bx] = P0[Bx];
-      U0[By] = P0[By];
-      U0[Bz] = P0[Bz];
-    }
+    prim_to_cons_ref(P0, U0, gamma);
     const double v2 = P0[vx]*P0[vx] + P0[vy]*P0[vy] + P0[vz]*P0[vz];
     const double B2 = P0[Bx]*P0[Bx] + P0[By]*P0[By] + P0[Bz]*P0[Bz];
     const double W0 = 1.0 / sqrt(1.0 - v2);
@@ -1163,6 +1191,184 @@ __global__ __launch_bounds__(kBlock) void k_measure_reduce(const double *__restr
     __syncthreads();
   }
   if (threadIdx.x == 0) out[q] = red[0];
+}
+
+/* ---------------------------------------------------------------------------
+ * Driving and cooling, applied to the committed state after a successful step as
+ * luaC_Mara_advance does (Mara/mara.cpp:330-333)
+ * ------------------------------------------------------------------------- */
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+/* DrivingProcedure::ResampleField (Mara/driving.cpp:44-85): the field at every zone centre,
+ * ghosts included, F(x) = Re sum_m F_m exp(i k_m . x), summed from the last mode to the
+ * first as the reference's valarray expression is. exp(i k . x) is the product of per-axis
+ * factors exp(2 pi i n x) from tables (ex, ey, ez: 2 k1 + 1 per zone index), which differs
+ * from the exponential of the summed phase by round-off. The amplitudes (3 numk complex
+ * numbers, axis-major) are staged in LDS. */
+__global__ __launch_bounds__(kBlock) void k_ou_sample(const double2 *__restrict__ ex, const double2 *__restrict__ ey,
+                                                      const double2 *__restrict__ ez, const double2 *__restrict__ amp,
+                                                      int k1, GridDesc g, double *__restrict__ Fd)
+{
+  extern __shared__ double2 sh_amp[];
+  const int nk = 2 * k1 + 1, numk = nk * nk * nk;
+  for (int t = threadIdx.x; t < 3 * numk; t += blockDim.x) sh_amp[t] = amp[t];
+  __syncthreads();
+  const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= g.T) return;
+  const long k = z % g.n[2], j = (z / g.n[2]) % g.n[1], i = z / ((long)g.n[1] * g.n[2]);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int a = nk - 1; a >= 0; --a) {
+    const double2 X = ex[i * nk + a];
+    for (int b = nk - 1; b >= 0; --b) {
+      const double2 xy = cmul(X, ey[j * nk + b]);
+      for (int c = nk - 1; c >= 0; --c) {
+        const double2 e = cmul(xy, ez[k * nk + c]);
+        const int m = (a * nk + b) * nk + c;
+        const double2 fx = sh_amp[m], fy = sh_amp[numk + m], fz = sh_amp[2 * numk + m];
+        s0 += fx.x * e.x - fx.y * e.y;
+        s1 += fy.x * e.x - fy.y * e.y;
+        s2 += fz.x * e.x - fz.y * e.y;
+      }
+    }
+  }
+  Fd[z] = s0;
+  Fd[g.T + z] = s1;
+  Fd[2 * g.T + z] = s2;
+}
+
+/* The field at n arbitrary points (xyz, 3 per point): the per-axis factors are formed per
+ * point (nk <= 13 per axis), then summed as in k_ou_sample */
+__global__ __launch_bounds__(kBlock) void kb_ou_sample(long n, const double *__restrict__ xyz,
+                                                       const double2 *__restrict__ amp, int k1, double *__restrict__ F)
+{
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int nk = 2 * k1 + 1, numk = nk * nk * nk;
+  double2 e3[3][2 * kOuMaxK1 + 1];
+  for (int d = 0; d < 3; ++d)
+    for (int a = 0; a < nk; ++a) {
+      const double ph = (2 * M_PI * (a - k1)) * xyz[3 * p + d];
+      double sn, cs;
+      sincos(ph, &sn, &cs);
+      e3[d][a] = make_double2(cs, sn);
+    }
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int a = nk - 1; a >= 0; --a)
+    for (int b = nk - 1; b >= 0; --b) {
+      const double2 xy = cmul(e3[0][a], e3[1][b]);
+      for (int c = nk - 1; c >= 0; --c) {
+        const double2 e = cmul(xy, e3[2][c]);
+        const int m = (a * nk + b) * nk + c;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) s[q] += amp[q * numk + m].x * e.x - amp[q * numk + m].y * e.y;
+      }
+    }
+  for (int q = 0; q < 3; ++q) F[3 * p + q] = s[q];
+}
+
+/* DrivingProcedure::Drive_rmhd (Mara/driving.cpp:154-186) on every zone: the force kicks
+ * the four-velocity, u += F dt / W, and rho is rescaled so that D = rho W is unchanged */
+__global__ __launch_bounds__(kBlock) void k_drive(double *__restrict__ P, const double *__restrict__ Fd, GridDesc g,
+                                                  double dt)
+{
+#pragma clang fp contract(off)
+  const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long T = g.T;
+  if (z >= T) return;
+  double uf[4] = {0.0, P[vx * T + z], P[vy * T + z], P[vz * T + z]};
+  const double v2 = uf[1] * uf[1] + uf[2] * uf[2] + uf[3] * uf[3];
+  const double W0 = 1.0 / sqrt(1.0 - v2);
+  uf[1] *= W0; uf[1] += Fd[z] * (dt / W0);
+  uf[2] *= W0; uf[2] += Fd[T + z] * (dt / W0);
+  uf[3] *= W0; uf[3] += Fd[2 * T + z] * (dt / W0);
+  const double u2 = uf[1] * uf[1] + uf[2] * uf[2] + uf[3] * uf[3];
+  const double W1 = sqrt(1.0 + u2);
+  P[vx * T + z] = uf[1] / W1;
+  P[vy * T + z] = uf[2] / W1;
+  P[vz * T + z] = uf[3] / W1;
+  P[rho * T + z] *= W0 / W1;
+}
+
+/* CoolingModuleT4 (kind 1) and CoolingModuleE4 (kind 2), Mara/cooling.cpp:13-103, on every
+ * zone: the temperature (T4, in MeV) or the specific internal energy (E4) loses
+ * dt (ref / t0) (x / ref)^4, the pressure is set from it, and the block's share of the
+ * energy removed, (tau_before - tau_after) dV, is written to part[blockIdx.x] (fixed-order
+ * block reduction) */
+__global__ __launch_bounds__(kBlock) void k_cool(double *__restrict__ P, GridDesc g, double gamma, int kind,
+                                                 double ref, double t0, double dt, double dV, MeasureUnits mu,
+                                                 double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+  const long T = g.T;
+  double removed = 0.0;
+  for (long z = (long)blockIdx.x * blockDim.x + threadIdx.x; z < T; z += (long)gridDim.x * blockDim.x) {
+    double P0[8], U0[8], U1[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P0[q] = P[q * T + z];
+    double T1;
+    if (kind == 1) {
+      const double p = P0[pre] / mu.ErgPerCC, r = P0[rho] / mu.GramsPerCC;   // TemperatureMeV
+      const double T0 = ((p / r) * (mu.mp / mu.kb)) * mu.K2MeV;
+      const double T1_MeV = T0 - dt * (ref / t0) * pow(T0 / ref, 4.0);
+      const double rr = P0[rho] / mu.GramsPerCC;                            // TemperatureArb
+      const double T_Kelvin = T1_MeV / mu.K2MeV;
+      T1 = ((mu.kb * T_Kelvin) * (rr / mu.mp)) * mu.ErgPerCC;
+    }
+    else {
+      const double e0 = (P0[pre] / (gamma - 1)) / P0[rho];
+      const double e1 = e0 - dt * (ref / t0) * pow(e0 / ref, 4.0);
+      T1 = (P0[rho] * e1) * (gamma - 1);
+    }
+    prim_to_cons_ref(P0, U0, gamma);
+    P0[pre] = T1;
+    prim_to_cons_ref(P0, U1, gamma);
+    removed += (U0[tau] - U1[tau]) * dV;
+    P[pre * T + z] = T1;
+  }
+  __shared__ double red[kBlock / 64];
+  const double w = wave_sum_d(removed);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+    for (int i = 1; i < kBlock / 64; ++i) t += red[i];
+    part[blockIdx.x] = t;
+  }
+}
+
+/* AveragePowerInFields' numerator: per block, the sum of |F|^2 over its zones */
+__global__ __launch_bounds__(kBlock) void k_power(const double *__restrict__ Fd, long T, double *__restrict__ part)
+{
+  double v = 0.0;
+  for (long z = (long)blockIdx.x * blockDim.x + threadIdx.x; z < T; z += (long)gridDim.x * blockDim.x)
+    v += Fd[z] * Fd[z] + Fd[T + z] * Fd[T + z] + Fd[2 * T + z] * Fd[2 * T + z];
+  __shared__ double red[kBlock / 64];
+  const double w = wave_sum_d(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = red[0];
+    for (int i = 1; i < kBlock / 64; ++i) t += red[i];
+    part[blockIdx.x] = t;
+  }
+}
+
+/* Adds the nb block sums, in a fixed order, to the running total *acc */
+__global__ __launch_bounds__(kBlock) void k_add_partials(const double *__restrict__ part, int nb, double *acc)
+{
+  __shared__ double red[kBlock];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) v += part[b];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[0] += red[0];
 }
 
 __global__ void k_divB(const double *__restrict__ P, GridDesc g, double *out /* sum, max, absmax */)
@@ -1303,6 +1509,16 @@ inline unsigned nblocks(long n) { return (unsigned)((n + kBlock - 1) / kBlock); 
 /* ============================================================================
  * context
  * ========================================================================== */
+namespace {
+/* a device buffer freed with its scope */
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) hipFree(p); }
+  template <typename T> T *get() { return static_cast<T *>(p); }
+};
+int dalloc(DevBuf &b, size_t bytes) { return hipMalloc(&b.p, bytes ? bytes : 1) == hipSuccess ? 0 : MARAGPU_E_NOMEM; }
+}  // namespace
+
 struct maragpu_ctx {
   maragpu_config cfg;
   GridDesc g;
@@ -1330,6 +1546,14 @@ struct maragpu_ctx {
   void *FZ = nullptr;
   // the measure table: per-block partials and the reduced values
   double *dmeas = nullptr;
+  // driving (DrivingProcedure): the field sampled at every zone, 3 planes of T doubles
+  double *Fdrive = nullptr;
+  // cooling (CoolingModuleT4 / E4): kind 0 none, 1 T4, 2 E4; per-block partials of the
+  // energy removed and, after them, its running total since the last read
+  int cool_kind = 0;
+  double cool_ref = 0.0, cool_t0 = 0.0;
+  double cool_units[3] = {1.0, 1.0, 1.0};
+  double *dcool = nullptr;
 };
 
 namespace {
@@ -1515,6 +1739,8 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->Uw);
   hipFree(c->FZ);
   hipFree(c->dmeas);
+  hipFree(c->Fdrive);
+  hipFree(c->dcool);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -1674,7 +1900,20 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   unsigned long long bits = h.lam_bits;
   std::memcpy(&lam, &bits, sizeof(lam));
   c->max_lambda = lam;
-  if (errors == 0) std::swap(c->Pc, c->Pw);
+  if (errors == 0) {
+    std::swap(c->Pc, c->Pw);
+    // source steps on the committed state (luaC_Mara_advance, Mara/mara.cpp:330-333)
+    if (c->Fdrive) k_drive<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pc, c->Fdrive, g, dt);
+    if (c->cool_kind) {
+      MeasureUnits mu = measure_units(c->cool_units);
+      const double dV = ((c->cfg.x1[0] - c->cfg.x0[0]) / g.N[0]) * ((c->cfg.x1[1] - c->cfg.x0[1]) / g.N[1]) *
+                        ((c->cfg.x1[2] - c->cfg.x0[2]) / g.N[2]);
+      k_cool<<<kCoolBlocks, kBlock, 0, c->stream>>>(c->Pc, g, c->cfg.gamma, c->cool_kind, c->cool_ref, c->cool_t0,
+                                                    dt, dV, mu, c->dcool);
+      k_add_partials<<<1, kBlock, 0, c->stream>>>(c->dcool, kCoolBlocks, c->dcool + kCoolBlocks);
+    }
+    if (c->Fdrive || c->cool_kind) HIP_CHECK(hipGetLastError());
+  }
 
   auto sum_ms = [&](const std::vector<EventPair> &v) {
     double t = 0.0;
@@ -1755,18 +1994,13 @@ int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
   if (!c || !out) return MARAGPU_E_INVALID;
   // PhysicalUnits(Length, Mass, Time), Mara/units.cpp:3-6 and :92-141; cgs by default
   // (HydroModule's units, Mara/hydro.cpp:20); constants of Mara/eos.hpp:12-15
-  const double Length = units ? units[0] : 1.0, Mass = units ? units[1] : 1.0, Time = units ? units[2] : 1.0;
-  if (!(Length > 0.0) || !(Mass > 0.0) || !(Time > 0.0)) return MARAGPU_E_INVALID;
-  const double Gram = 1.0 / Mass, Centimeter = 1.0 / Length, Second = 1.0 / Time;
-  const double Erg = Gram * std::pow(Centimeter / Second, 2.0);
-  const double MeV = 1.60217648740e-06 * Erg;
+  const double cgs[3] = {1.0, 1.0, 1.0};
+  if (!units) units = cgs;
+  if (!(units[0] > 0.0) || !(units[1] > 0.0) || !(units[2] > 0.0)) return MARAGPU_E_INVALID;
+  const double Centimeter = 1.0 / units[0], Second = 1.0 / units[2];
+  const double Erg = (1.0 / units[1]) * std::pow(Centimeter / Second, 2.0);
   const double Gauss = std::sqrt(Erg / std::pow(Centimeter, 3.0) / (4 * M_PI));
-  MeasureUnits mu;
-  mu.GramsPerCC = Gram / std::pow(Centimeter, 3.0);
-  mu.ErgPerCC = MeV / std::pow(1e-13 * Centimeter, 3) * 1e-33;
-  mu.mp = 1.67262177774e-24 * Gram;
-  mu.kb = 1.38064881300e-16 * Erg;
-  mu.K2MeV = 8.621738e-11;
+  const MeasureUnits mu = measure_units(units);
   if (!c->dmeas && hipMalloc(&c->dmeas, (size_t)(kMeasBlocks + 1) * kMeasQ * sizeof(double)) != hipSuccess) {
     c->dmeas = nullptr;
     return MARAGPU_E_NOMEM;
@@ -1795,6 +2029,110 @@ int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
   out->mean_max_magnetic_field[0] = (1.0 / Gauss) * r[27] / ttl;
   out->mean_max_magnetic_field[1] = (1.0 / Gauss) * r[kMeasSums + 3];
   out->max_lorentz_factor = r[kMeasSums + 4];
+  return MARAGPU_OK;
+}
+
+namespace {
+/* per-axis tables exp(2 pi i n x) at the zone centres x_at (Mara/simple-cart.hpp:42-44) for
+ * n = -k1..k1, and the amplitudes (axis-major), on the device */
+int ou_tables(const maragpu_host::OuField &f, const GridDesc &g, const maragpu_config &cfg, DevBuf &tab,
+              DevBuf &amp)
+{
+  const int nk = 2 * f.k1 + 1;
+  std::vector<double2> h;
+  for (int d = 0; d < 3; ++d) {
+    const double dx = (cfg.x1[d] - cfg.x0[d]) / cfg.N[d];
+    for (int i = 0; i < g.n[d]; ++i) {
+      const double x = cfg.x0[d] + dx * (i - g.Ng + 0.5);
+      for (int a = 0; a < nk; ++a) {
+        const double ph = (2 * M_PI * (a - f.k1)) * x;
+        h.push_back(make_double2(std::cos(ph), std::sin(ph)));
+      }
+    }
+  }
+  std::vector<double2> ha(3 * (size_t)f.numk);
+  for (size_t q = 0; q < ha.size(); ++q) ha[q] = make_double2(f.F[2 * q], f.F[2 * q + 1]);
+  if (dalloc(tab, h.size() * sizeof(double2)) || dalloc(amp, ha.size() * sizeof(double2))) return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(tab.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(amp.p, ha.data(), ha.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return MARAGPU_OK;
+}
+}  // namespace
+
+int maragpu_set_driving(maragpu_ctx *c, const maragpu_ou_field *field)
+{
+  if (!c) return MARAGPU_E_INVALID;
+  if (!field) {
+    hipFree(c->Fdrive);
+    c->Fdrive = nullptr;
+    return MARAGPU_OK;
+  }
+  const maragpu_host::OuField &f = *maragpu_ou_field_modes(field);
+  if (f.k1 > kOuMaxK1) return MARAGPU_E_INVALID;
+  const GridDesc &g = c->g;
+  if (!c->Fdrive && hipMalloc(&c->Fdrive, 3 * (size_t)g.T * sizeof(double)) != hipSuccess) {
+    c->Fdrive = nullptr;
+    return MARAGPU_E_NOMEM;
+  }
+  DevBuf tab, amp;
+  if (int rc = ou_tables(f, g, c->cfg, tab, amp)) return rc;
+  const int nk = 2 * f.k1 + 1;
+  const double2 *t = tab.get<double2>();
+  k_ou_sample<<<nblocks(g.T), kBlock, 3 * f.numk * sizeof(double2), c->stream>>>(
+      t, t + (size_t)g.n[0] * nk, t + (size_t)(g.n[0] + g.n[1]) * nk, amp.get<double2>(), f.k1, g, c->Fdrive);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MARAGPU_OK;
+}
+
+int maragpu_driving_power(maragpu_ctx *c, double *power)
+{
+  if (!c || !power) return MARAGPU_E_INVALID;
+  if (!c->Fdrive) { *power = 0.0; return MARAGPU_OK; }
+  // AveragePowerInFields (Mara/driving.cpp:30-39): mean of |F|^2 over every zone
+  DevBuf part;
+  if (dalloc(part, (kCoolBlocks + 1) * sizeof(double))) return MARAGPU_E_NOMEM;
+  double *acc = part.get<double>() + kCoolBlocks;
+  const double zero = 0.0;
+  HIP_CHECK(hipMemcpyAsync(acc, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_power<<<kCoolBlocks, kBlock, 0, c->stream>>>(c->Fdrive, c->g.T, part.get<double>());
+  k_add_partials<<<1, kBlock, 0, c->stream>>>(part.get<double>(), kCoolBlocks, acc);
+  double s = 0.0;
+  HIP_CHECK(hipMemcpyAsync(&s, acc, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  *power = s / c->g.T;
+  return MARAGPU_OK;
+}
+
+int maragpu_set_cooling(maragpu_ctx *c, int kind, double ref, double t0, const double *units)
+{
+  if (!c || kind < 0 || kind > 2) return MARAGPU_E_INVALID;
+  if (kind && (!(ref > 0.0) || !(t0 > 0.0))) return MARAGPU_E_INVALID;
+  if (units && (!(units[0] > 0.0) || !(units[1] > 0.0) || !(units[2] > 0.0))) return MARAGPU_E_INVALID;
+  if (kind && !c->dcool) {
+    if (hipMalloc(&c->dcool, (kCoolBlocks + 1) * sizeof(double)) != hipSuccess) {
+      c->dcool = nullptr;
+      return MARAGPU_E_NOMEM;
+    }
+    const double zero = 0.0;
+    HIP_CHECK(hipMemcpy(c->dcool + kCoolBlocks, &zero, sizeof(double), hipMemcpyHostToDevice));
+  }
+  c->cool_kind = kind;
+  c->cool_ref = ref;
+  c->cool_t0 = t0;
+  for (int d = 0; d < 3; ++d) c->cool_units[d] = units ? units[d] : 1.0;
+  return MARAGPU_OK;
+}
+
+int maragpu_cooling_energy_removed(maragpu_ctx *c, double *removed)
+{
+  if (!c || !removed) return MARAGPU_E_INVALID;
+  *removed = 0.0;
+  if (!c->dcool) return MARAGPU_OK;
+  const double zero = 0.0;
+  HIP_CHECK(hipMemcpyAsync(removed, c->dcool + kCoolBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipMemcpyAsync(c->dcool + kCoolBlocks, &zero, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
   return MARAGPU_OK;
 }
 
@@ -1837,14 +2175,6 @@ const double *maragpu_committed_prim(maragpu_ctx *c, int *N, int *Ng, long *T, i
 /* ---------------------------------------------------------------------------
  * batch entry points
  * ------------------------------------------------------------------------- */
-namespace {
-struct DevBuf {
-  void *p = nullptr;
-  ~DevBuf() { if (p) hipFree(p); }
-  template <typename T> T *get() { return static_cast<T *>(p); }
-};
-int dalloc(DevBuf &b, size_t bytes) { return hipMalloc(&b.p, bytes ? bytes : 1) == hipSuccess ? 0 : MARAGPU_E_NOMEM; }
-}  // namespace
 
 extern "C" {
 
@@ -1896,6 +2226,24 @@ int maragpu_batch_flux_eigen(size_t n, const double *P, double *F, double *ap, d
   HIP_CHECK(hipMemcpy(F, dF.p, n * 64, hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(ap, dA.p, n * 8, hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(am, dB.p, n * 8, hipMemcpyDeviceToHost));
+  return MARAGPU_OK;
+}
+
+int maragpu_ou_field_sample(const maragpu_ou_field *field, size_t n, const double *xyz, double *F)
+{
+  if (!field || !xyz || !F) return MARAGPU_E_INVALID;
+  const maragpu_host::OuField &f = *maragpu_ou_field_modes(field);
+  if (f.k1 > kOuMaxK1) return MARAGPU_E_INVALID;
+  if (n == 0) return MARAGPU_OK;
+  std::vector<double2> ha(3 * (size_t)f.numk);
+  for (size_t q = 0; q < ha.size(); ++q) ha[q] = make_double2(f.F[2 * q], f.F[2 * q + 1]);
+  DevBuf dX, dF, dA;
+  if (dalloc(dX, n * 24) || dalloc(dF, n * 24) || dalloc(dA, ha.size() * sizeof(double2))) return MARAGPU_E_NOMEM;
+  HIP_CHECK(hipMemcpy(dX.p, xyz, n * 24, hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(dA.p, ha.data(), ha.size() * sizeof(double2), hipMemcpyHostToDevice));
+  kb_ou_sample<<<nblocks(n), kBlock>>>((long)n, dX.get<double>(), dA.get<double2>(), f.k1, dF.get<double>());
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpy(F, dF.p, n * 24, hipMemcpyDeviceToHost));
   return MARAGPU_OK;
 }
 

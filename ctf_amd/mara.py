@@ -89,8 +89,53 @@ class _Measure:
         return x.value
 
 
+class _Driving:
+    """Mara.driving (Mara/mara.cpp:219-224)"""
+
+    def Advance(self, dt):
+        _chk(native.lib().mara_driving_advance(float(dt)))
+
+    def Resample(self):
+        _chk(native.lib().mara_driving_resample())
+
+    def Serialize(self):
+        n = C.c_size_t()
+        _chk(native.lib().mara_driving_serialize(None, 0, C.byref(n)))
+        if n.value == 0:
+            return None
+        buf = C.create_string_buffer(n.value)
+        _chk(native.lib().mara_driving_serialize(buf, n.value, C.byref(n)))
+        return buf.raw
+
+    def Power(self):
+        p = C.c_double()
+        _chk(native.lib().mara_driving_power(C.byref(p)))
+        return p.value
+
+
 class _Mara:
     measure = _Measure()
+    driving = _Driving()
+
+    def new_ou_field(self, Nd, P0, zeta, k1, seed):
+        n = C.c_size_t()
+        _chk(native.lib().mara_new_ou_field(int(Nd), float(P0), float(zeta), int(k1), int(seed), None, 0,
+                                            C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        _chk(native.lib().mara_new_ou_field(int(Nd), float(P0), float(zeta), int(k1), int(seed), buf, n.value,
+                                            C.byref(n)))
+        return buf.raw
+
+    def set_driving(self, field: bytes):
+        _chk(native.lib().mara_set_driving(field, len(field)))
+
+    def set_cooling(self, key, ref, t0):
+        _chk(native.lib().mara_set_cooling(_s(key), float(ref), float(t0)))
+
+    def cooling_rate(self, dt):
+        r = C.c_double()
+        _chk(native.lib().mara_cooling_rate(float(dt), C.byref(r)))
+        return r.value
 
     def start(self):
         _chk(native.lib().mara_start())

@@ -42,6 +42,11 @@ EXPORTS = [
     "mara_measure_mean_prim", "mara_measure_mean_energies", "mara_measure_mean_max_temperature",
     "mara_measure_mean_max_sonic_mach", "mara_measure_mean_min_alfvenic_mach",
     "mara_measure_mean_max_magnetic_field", "mara_measure_max_lorentz_factor",
+    "maragpu_ou_field_create", "maragpu_ou_field_load", "maragpu_ou_field_bytes", "maragpu_ou_field_save",
+    "maragpu_ou_field_advance", "maragpu_ou_field_destroy", "maragpu_ou_field_sample", "maragpu_set_driving",
+    "maragpu_driving_power", "maragpu_set_cooling", "maragpu_cooling_energy_removed",
+    "mara_new_ou_field", "mara_set_driving", "mara_driving_advance", "mara_driving_resample",
+    "mara_driving_serialize", "mara_driving_power", "mara_set_cooling", "mara_cooling_rate",
     "mara_device_context", "mara_init_turbulence",
     "maragpu_mlp_create", "maragpu_mlp_destroy", "maragpu_mlp_param_count",
     "maragpu_mlp_serialized_size", "maragpu_mlp_save", "maragpu_mlp_load", "maragpu_mlp_get_params",
@@ -168,6 +173,26 @@ def lib():
         "mara_measure_mean_min_alfvenic_mach": (C.c_int, [_dp, _dp]),
         "mara_measure_mean_max_magnetic_field": (C.c_int, [_dp, _dp]),
         "mara_measure_max_lorentz_factor": (C.c_int, [_dp]),
+        "maragpu_ou_field_create": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, C.POINTER(_vp)]),
+        "maragpu_ou_field_load": (C.c_int, [C.c_char_p, _sz, C.POINTER(_vp)]),
+        "maragpu_ou_field_bytes": (_sz, [_vp]),
+        "maragpu_ou_field_save": (C.c_int, [_vp, C.c_char_p, _sz]),
+        "maragpu_ou_field_advance": (C.c_int, [_vp, C.c_double]),
+        "maragpu_ou_field_destroy": (C.c_int, [_vp]),
+        "maragpu_ou_field_sample": (C.c_int, [_vp, _sz, _dp, _dp]),
+        "maragpu_set_driving": (C.c_int, [_vp, _vp]),
+        "maragpu_driving_power": (C.c_int, [_vp, _dp]),
+        "maragpu_set_cooling": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, _dp]),
+        "maragpu_cooling_energy_removed": (C.c_int, [_vp, _dp]),
+        "mara_new_ou_field": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_char_p, _sz,
+                                        C.POINTER(_sz)]),
+        "mara_set_driving": (C.c_int, [C.c_char_p, _sz]),
+        "mara_driving_advance": (C.c_int, [C.c_double]),
+        "mara_driving_resample": (C.c_int, []),
+        "mara_driving_serialize": (C.c_int, [C.c_char_p, _sz, C.POINTER(_sz)]),
+        "mara_driving_power": (C.c_int, [_dp]),
+        "mara_set_cooling": (C.c_int, [C.c_char_p, C.c_double, C.c_double]),
+        "mara_cooling_rate": (C.c_int, [C.c_double, _dp]),
         "mara_last_error": (C.c_char_p, []),
         "mara_device_context": (_vp, []),
         "mara_init_turbulence": (C.c_int, [C.POINTER(ICParams), _dp, _sz]),
@@ -320,6 +345,25 @@ class Context:
         check(lib().maragpu_measure(self.h, u, C.byref(m)), "measure")
         return m.as_dict()
 
+    def set_driving(self, field: "OuField"):
+        """Samples the field over the grid and drives after every successful advance."""
+        check(lib().maragpu_set_driving(self.h, field.h if field is not None else None), "set_driving")
+
+    def driving_power(self):
+        p = C.c_double()
+        check(lib().maragpu_driving_power(self.h, C.byref(p)), "driving_power")
+        return p.value
+
+    def set_cooling(self, kind, ref=0.0, t0=0.0, units=None):
+        k = {None: 0, "T4": 1, "E4": 2}[kind]
+        u = None if units is None else (C.c_double * 3)(*[float(x) for x in units])
+        check(lib().maragpu_set_cooling(self.h, k, ref, t0, u), "set_cooling")
+
+    def cooling_energy_removed(self):
+        e = C.c_double()
+        check(lib().maragpu_cooling_energy_removed(self.h, C.byref(e)), "cooling_energy_removed")
+        return e.value
+
     def attach_mlp(self, net: "Mlp"):
         """Use `net` for the learned c2p guess (the context keeps a reference)."""
         self._mlp = net
@@ -421,6 +465,52 @@ def init_turbulence(N, Ng=3, rho=1.0, pre=1.0, B0=(0.0, 0.0, 1.0), v_rms=0.3, v_
 # ---------------------------------------------------------------------------
 # learned c2p network
 # ---------------------------------------------------------------------------
+class OuField:
+    """StochasticVectorField3d on the host (maragpu_ou_field): create(P0, zeta, k1, seed),
+    or load(bytes) in the reference's serialized layout."""
+
+    def __init__(self, h):
+        self.h = h
+
+    @classmethod
+    def create(cls, P0, zeta, k1, seed):
+        h = _vp()
+        check(lib().maragpu_ou_field_create(P0, zeta, k1, seed, C.byref(h)), "ou_field_create")
+        return cls(h)
+
+    @classmethod
+    def load(cls, data: bytes):
+        h = _vp()
+        check(lib().maragpu_ou_field_load(data, len(data), C.byref(h)), "ou_field_load")
+        return cls(h)
+
+    def bytes(self) -> bytes:
+        n = lib().maragpu_ou_field_bytes(self.h)
+        buf = C.create_string_buffer(n)
+        check(lib().maragpu_ou_field_save(self.h, buf, n), "ou_field_save")
+        return buf.raw
+
+    def advance(self, dt):
+        check(lib().maragpu_ou_field_advance(self.h, dt), "ou_field_advance")
+
+    def sample(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        F = np.zeros_like(xyz)
+        check(lib().maragpu_ou_field_sample(self.h, len(xyz), dptr(xyz), dptr(F)), "ou_field_sample")
+        return F
+
+    def close(self):
+        if self.h:
+            lib().maragpu_ou_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Mlp:
     """The learned c2p network (maragpu_mlp): device-resident parameters."""
 

@@ -27,6 +27,11 @@
  *   mara_measure_mean_velocity, _mean_cons, _mean_prim, _mean_energies, _mean_max_temperature,
  *   _mean_max_sonic_mach, _mean_min_alfvenic_mach, _mean_max_magnetic_field, _max_lorentz_factor
  *                                    the Mara.measure table          (:1954-2262)
+ *   mara_new_ou_field                luaC_new_ou_field               (:1378-1400) -> bytes
+ *   mara_set_driving                 luaC_set_driving                (:1169-1196)
+ *   mara_driving_{advance,resample,serialize,power}  Mara.driving.*  (:1771-1808)
+ *   mara_set_cooling                 luaC_set_cooling                (:1198-1217) 'T4' 'E4'
+ *   mara_cooling_rate                luaC_cooling_rate               (:1297-1309)
  *   mara_init_turbulence             (synthetic data; the role of an init_prim callback, :531-632)
  */
 #ifndef MARA_HOST_H
@@ -73,6 +78,19 @@ int mara_measure_mean_max_sonic_mach(double *mean, double *max);
 int mara_measure_mean_min_alfvenic_mach(double *mean, double *min);
 int mara_measure_mean_max_magnetic_field(double *mean, double *max);
 int mara_measure_max_lorentz_factor(double *max);
+/* Driving and cooling. A serialized field is the reference's byte string: new_ou_field and
+ * driving_serialize write it into buf when cap suffices and always report its length;
+ * set_driving samples it over the domain (on the device), driving_advance steps the
+ * field's OU processes, driving_resample samples the advanced field. Driving and then
+ * cooling act on the state after every successful advance. */
+int mara_new_ou_field(int Nd, double P0, double zeta, int k1, int seed, void *buf, size_t cap, size_t *len);
+int mara_set_driving(const void *bytes, size_t n);
+int mara_driving_advance(double dt);
+int mara_driving_resample(void);
+int mara_driving_serialize(void *buf, size_t cap, size_t *len);
+int mara_driving_power(double *power);
+int mara_set_cooling(const char *key, double ref, double t0);
+int mara_cooling_rate(double dt, double *rate);   /* energy removed since the last call / dt */
 const char *mara_last_error(void);
 /* the device context behind the application (created on first use), for callers that
  * want to keep the state on the device between steps */

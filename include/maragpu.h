@@ -21,6 +21,8 @@
  *   maragpu_get_timestep               luaC_Mara_get_timestep (Mara/mara.cpp:374-381)
  *   maragpu_measure_divB               luaC_mean_max_divB (Mara/mara.cpp:2264-2324)
  *   maragpu_measure                    the other Mara.measure functions (Mara/mara.cpp:1954-2262)
+ *   maragpu_ou_field_*, maragpu_set_driving, maragpu_set_cooling
+ *                                      StochasticVectorField3d, DrivingProcedure, CoolingModuleT4/E4
  *   maragpu_batch_prim_to_cons         AdiabaticIdealRmhd::PrimToCons (Mara/rmhd.cpp:133-163)
  *   maragpu_batch_cons_to_prim         AdiabaticIdealRmhd::ConsToPrim cascade (Mara/rmhd.cpp:59-132)
  *                                      over the rmhd-c2p.h C API (Mara/rmhd-c2p.h:22-35)
@@ -127,6 +129,34 @@ typedef struct maragpu_measures {
   double max_lorentz_factor;         /*                               luaC_max_lorentz_factor (:2232-2258) */
 } maragpu_measures;
 int maragpu_measure(maragpu_ctx *ctx, const double *units, maragpu_measures *out);
+/* ---------------------------------------------------------------------------------
+ * Driving and cooling: source steps applied to the committed state after every
+ * successful advance, as luaC_Mara_advance does (Mara/mara.cpp:330-333).
+ * ------------------------------------------------------------------------------- */
+/* StochasticVectorField3d (Mara/ou-field.cpp:44-74): Fourier modes |k_i| <= k1 driven by
+ * Ornstein-Uhlenbeck processes, with the spectrum of luaC_new_ou_field (Mara/mara.cpp:1373-1394).
+ * Held on the host; bytes are the reference's Serialize layout (:261-278), so
+ * Mara.new_ou_field output and Mara checkpoints load unchanged. */
+typedef struct maragpu_ou_field maragpu_ou_field;
+int maragpu_ou_field_create(double P0, double zeta, int k1, int seed, maragpu_ou_field **out);
+int maragpu_ou_field_load(const void *bytes, size_t n, maragpu_ou_field **out);
+size_t maragpu_ou_field_bytes(const maragpu_ou_field *field);
+int maragpu_ou_field_save(const maragpu_ou_field *field, void *bytes, size_t n);
+int maragpu_ou_field_advance(maragpu_ou_field *field, double dt);          /* AdvanceField (:186-210) */
+int maragpu_ou_field_destroy(maragpu_ou_field *field);
+/* SampleField (:226-239) at n points xyz (3 doubles each) on the device; F: 3 per point */
+int maragpu_ou_field_sample(const maragpu_ou_field *field, size_t n, const double *xyz, double *F);
+/* DrivingProcedure (Mara/driving.cpp): samples the field at every zone of the context
+ * (ResampleField, :44-85; k1 <= 6) and drives the committed state after each successful
+ * advance (Drive_rmhd, :154-186). NULL turns driving off. Call again to resample. */
+int maragpu_set_driving(maragpu_ctx *ctx, const maragpu_ou_field *field);
+int maragpu_driving_power(maragpu_ctx *ctx, double *power);               /* AveragePowerInFields */
+/* CoolingModuleT4 (kind 1) / CoolingModuleE4 (kind 2) (Mara/cooling.cpp), 0 = none; units =
+ * {Length, Mass, Time} (NULL: cgs) */
+int maragpu_set_cooling(maragpu_ctx *ctx, int kind, double ref, double t0, const double *units);
+/* EnergyRemoved: the energy taken out since the last call (then reset) */
+int maragpu_cooling_energy_removed(maragpu_ctx *ctx, double *removed);
+
 /* device time of the dominant kernel over the last advance (ms, per launch) and launches */
 int maragpu_kernel_times(maragpu_ctx *ctx, double *flux_ms, int *flux_launches, double *c2p_ms,
                          int *c2p_launches, double *update_ms, int *update_launches);
