@@ -172,6 +172,62 @@ def cpu_baseline(w, seconds_target=15.0):
                       f"{el:.1f} s single-threaded on the GPU box's host"}
 
 
+def driving_record(w, steps, k1=3):
+    """Driving and cooling at the workload's grid: the OU field of ctf-core/drvtrb.lua
+    (k1 = 3, 343 modes) sampled at every zone on the device (ResampleField), and the cost of
+    the drive and T4 cooling source steps per advance (plain form, with against without).
+    Beside it, the reference's own ResampleField (oracle/_ref, one thread) on a bounded
+    32^3 sample, scaled to zones per second."""
+    f = native.OuField.create(0.8, 0.2, k1, 12345)
+    f.advance(0.01)
+    P0 = native.init_turbulence(**w.ic_kwargs())
+    ctx = native.Context(native.make_config(**w.config_kwargs()))
+    ctx.upload(P0)
+    ctx.set_driving(f)                                   # first call allocates the planes
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        ctx.set_driving(f)
+    sample_s = (time.perf_counter() - t0) / reps
+    T = (w.N + 2 * w.Ng) ** 3
+    e, _ = ctx.advance(0.0)
+    dt = ctx.get_timestep(w.cfl)
+
+    def step_ms(n):
+        t = time.perf_counter()
+        for _ in range(n):
+            err, _ = ctx.advance(dt)
+            if err:
+                raise RuntimeError("driving record: step failed")
+        return 1e3 * (time.perf_counter() - t) / n
+
+    with_src = None
+    ctx.set_cooling("T4", 1e-18, 100.0)
+    ctx.advance(dt)
+    with_src = step_ms(steps)
+    ctx.set_driving(None)
+    ctx.set_cooling(None)
+    without = step_ms(steps)
+    ctx.close()
+    rec = {"grid": w.N, "modes": (2 * k1 + 1) ** 3, "zones_sampled": T,
+           "resample_ms": 1e3 * sample_s, "resample_zones_per_s": T / sample_s,
+           "step_ms_with_drive_and_cool": with_src, "step_ms_without": without,
+           "source_steps_ms": with_src - without}
+    sys.path.insert(0, ROOT)
+    from oracle.oracle import Lib, Setup, have_ref
+    if have_ref():
+        n = 32
+        lib = Lib("ref")
+        so = lib.solver(Setup(N=(n,) * 3))
+        P = native.init_turbulence(**w.resized(n).ic_kwargs())
+        t = time.perf_counter()
+        lib.drive(so, f.bytes(), P, 0.0)                 # DrivingProcedure samples every zone
+        el = time.perf_counter() - t
+        rec["cpu_reference_resample_zones_per_s"] = (n + 2 * w.Ng) ** 3 / el
+        rec["cpu_sample"] = f"reference DrivingProcedure at {n}^3 ({(n + 6) ** 3} zones), {el:.1f} s, one thread"
+    return rec
+
+
 def weno_split_record(w, N, steps):
     """The weno-split operator (local Lax-Friedrichs, finite-difference WENO5, RK3) on the
     same turbulence at N^3: component-wise and RMHD-characteristic projections, FP64 and
@@ -224,6 +280,7 @@ def main():
     ap.add_argument("--weno-split-grid", type=int, default=128,
                     help="grid of the weno-split measurement (0: skip it)")
     ap.add_argument("--weno-split-steps", type=int, default=3)
+    ap.add_argument("--no-driving", action="store_true", help="skip the driving/cooling record")
     ap.add_argument("--model", default=os.path.join(ROOT, "ctf_amd", "models", "c2p_mlp.bin"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the "
@@ -315,6 +372,10 @@ def main():
     ws = None
     if args.weno_split_grid:
         ws = weno_split_record(w, args.weno_split_grid, args.weno_split_steps)
+    dr = None
+    if not args.no_driving:
+        log("[bench] driving and cooling")
+        dr = driving_record(w, 3)
 
     out = {
         "metric": METRIC,
@@ -339,6 +400,8 @@ def main():
     }
     if ws:
         out["weno_split"] = ws
+    if dr:
+        out["driving"] = dr
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline")
         cb = cpu_baseline(w)
