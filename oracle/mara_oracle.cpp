@@ -362,16 +362,146 @@ void constrained_transport_3d(const Grid &g, double *Fx, double *Fy, double *Fz)
 
 /* MethodOfLinesSplit::dUdt, Mara/plm-split.cpp:11-33 and drive_sweeps_3d :103-120.
  * Returns false on IntermediateFailure (c2p failed somewhere). */
-bool dUdt(Grid &g, const std::vector<double> &Uin, std::vector<double> &L, Acc &acc, int *err)
+/* PlmCtuHancockOperator::ctu_hancock (Mara/ctu-hancock.cpp:71-494) in 3-D: its own PLM
+ * limiter (plm_minmod with the operator's theta, :22-31) on the stage primitives; Godunov
+ * fluxes of the half-slope states (step 2); a Hancock predictor per axis with the
+ * transverse Godunov fluxes (step 3); the predictors synchronised and inverted with P as
+ * the guess; the corrector fluxes from them over [sx, stride0 - sx) (step 4). With
+ * DoNotTolerateFailures (the reference never lowers it) every unphysical state or failed
+ * inversion adds one to the failure mask, and the step ends at the next abort point.
+ * Returns the mask's sum there, 0 when F, G, H hold the corrector fluxes. */
+int ctu_hancock(Grid &g, const double *P, const double *U, double *F, double *G, double *H, double dt, Acc &acc)
+{
+  const long s0 = 8 * g.T, sx = g.sx, sy = g.sy, sz = g.sz;
+  const double theta = g.cfg.theta;
+  auto sgn = [](double x) { return (double)((x > 0) - (x < 0)); };
+  auto minmod = [&](double ul, double u0, double ur) {
+    const double a = theta * (u0 - ul);
+    const double b =   0.5 * (ur - ul);
+    const double c = theta * (ur - u0);
+    const double ab = std::fabs(a) < std::fabs(b) ? std::fabs(a) : std::fabs(b);
+    const double m3 = ab < std::fabs(c) ? ab : std::fabs(c);
+    return 0.25*std::fabs(sgn(a) + sgn(b))*(sgn(a) + sgn(c))*m3;
+  };
+  std::vector<double> dPdx(s0, 0.0), dPdy(s0, 0.0), dPdz(s0, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long i = sx; i < s0 - sx; ++i) {
+    dPdx[i] = minmod(P[i-sx], P[i], P[i+sx]);
+    dPdy[i] = minmod(P[i-sy], P[i], P[i+sy]);
+    dPdz[i] = minmod(P[i-sz], P[i], P[i+sz]);
+  }
+  const double *dP[3] = {dPdx.data(), dPdy.data(), dPdz.data()};
+  double *Fd[3] = {F, G, H};
+  const long Sd[3] = {sx, sy, sz};
+  std::vector<int> mask(g.T, 0);
+  auto mask_sum = [&]() { long t = 0; for (int m : mask) t += m; return (int)t; };
+  double lam = 0.0;
+  long succ = 0, fail = 0, iters = 0;
+  // Godunov fluxes of reconstructed states: step 2 from P over [0, s0 - sx), step 4 from the
+  // predicted primitives over [sx, s0 - sx)
+  auto faces = [&](const double *const Psrc[3], long begin) {
+#pragma omp parallel for schedule(dynamic, 64) reduction(max : lam) reduction(+ : succ, fail, iters)
+    for (long i = begin; i < s0 - sx; i += 8) {
+      for (int d = 0; d < 3; ++d) {
+        const long S = Sd[d];
+        double Pl[8], Pr[8];
+        for (int j = 0; j < 8; ++j) {
+          Pr[j] = Psrc[d][i+S+j] - 0.5*dP[d][i+S+j];
+          Pl[j] = Psrc[d][i+j] + 0.5*dP[d][i+j];
+        }
+        if (c2p_check_prim(Pl) || c2p_check_prim(Pr)) {
+#pragma omp atomic
+          mask[i/8] += 1;
+          continue;
+        }
+        Tally<double> t;
+        riemann_flux(g.cfg.riemann, Pl, Pr, Fd[d] + i, d + 1, g.pp, t);
+        if (t.max_lambda > lam) lam = t.max_lambda;
+        succ += t.hlld_success;
+        fail += t.hlld_failure;
+        iters += t.c2p_iterations;
+      }
+    }
+  };
+  auto collect = [&]() {
+    if (lam > acc.max_lambda) acc.max_lambda = lam;
+    acc.hlld_success += succ; acc.hlld_failure += fail; acc.c2p_iter += iters;
+    lam = 0.0; succ = fail = iters = 0;
+  };
+  const double *Pthree[3] = {P, P, P};
+  faces(Pthree, 0);
+  collect();
+  if (int e = mask_sum()) return e;
+  // step 3: the Hancock predictor of each axis
+  std::vector<double> Ux(s0, 0.0), Uy(s0, 0.0), Uz(s0, 0.0);
+  double *Ud[3] = {Ux.data(), Uy.data(), Uz.data()};
+  const double dx = g.dx, dy = g.dy, dz = g.dz;
+#pragma omp parallel for schedule(static)
+  for (long i = sx; i < s0; i += 8) {
+    double Fo[3][2][8];                    // (FL, FR), (GL, GR), (HL, HR)
+    bool bad = false;
+    for (int d = 0; d < 3; ++d) {
+      double PL[8], PR[8], UL[8], UR[8];
+      for (int j = 0; j < 8; ++j) {
+        PL[j] = P[i+j] - 0.5*dP[d][i+j];
+        PR[j] = P[i+j] + 0.5*dP[d][i+j];
+      }
+      if (c2p_check_prim(PL) || c2p_check_prim(PR)) {
+        mask[i/8] += 1;
+        bad = true;
+        continue;
+      }
+      prim_to_cons(PL, UL, g.pp.gamma);
+      prim_to_cons(PR, UR, g.pp.gamma);
+      flux_and_eigenvalues<double>(UL, PL, Fo[d][0], nullptr, nullptr, d + 1, g.pp.gamma);
+      flux_and_eigenvalues<double>(UR, PR, Fo[d][1], nullptr, nullptr, d + 1, g.pp.gamma);
+    }
+    if (bad) continue;
+    for (int j = 0; j < 8; ++j) {
+      const double *FL = Fo[0][0], *FR = Fo[0][1], *GL = Fo[1][0], *GR = Fo[1][1], *HL = Fo[2][0], *HR = Fo[2][1];
+      Ux[i+j] = U[i+j] - ((FR[j]-FL[j])/dx + (G[i+j]-G[i-sy+j])/dy + (H[i+j]-H[i-sz+j])/dz)*0.5*dt;
+      Uy[i+j] = U[i+j] - ((GR[j]-GL[j])/dy + (H[i+j]-H[i-sz+j])/dz + (F[i+j]-F[i-sx+j])/dx)*0.5*dt;
+      Uz[i+j] = U[i+j] - ((HR[j]-HL[j])/dz + (F[i+j]-F[i-sx+j])/dx + (G[i+j]-G[i-sy+j])/dy)*0.5*dt;
+    }
+  }
+  if (int e = mask_sum()) return e;
+  // the predictors, synchronised and inverted from P (Rmhd::ConsToPrim, no emergency reset)
+  std::vector<double> Px(P, P + s0), Py(P, P + s0), Pz(P, P + s0);
+  double *Pd[3] = {Px.data(), Py.data(), Pz.data()};
+  for (int d = 0; d < 3; ++d) synchronize(g, Ud[d]);
+  long c2p_it = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : c2p_it)
+  for (long z = 0; z < g.T; ++z) {
+    for (int d = 0; d < 3; ++d) {
+      std::int64_t it = 0;
+      if (cons_to_prim(Ud[d] + 8*z, Pd[d] + 8*z, g.pp.gamma, g.pp.pfloor, &it) != 0) mask[z] += 1;
+      c2p_it += it;
+    }
+  }
+  acc.c2p_iter += c2p_it;
+  if (int e = mask_sum()) return e;
+  const double *Ppred[3] = {Px.data(), Py.data(), Pz.data()};
+  faces(Ppred, sx);
+  collect();
+  return mask_sum();
+}
+
+bool dUdt(Grid &g, const std::vector<double> &Uin, std::vector<double> &L, Acc &acc, int *err, double dt)
 {
   std::vector<double> U = Uin;
   L.assign(U.size(), 0.0);
   std::vector<double> &P = g.prim_array;
   int e = cons_to_prim_all(g, U.data(), P.data(), acc);
-  if (e != 0) { *err = e; return false; }
+  // PlmCtuHancockOperator::dUdt (Mara/ctu-hancock.cpp:52-69) ignores the count: failed zones
+  // keep their previous primitives and the operator restarts the failure mask
+  if (e != 0 && g.cfg.godunov != 2) { *err = e; return false; }
   const long s0 = 8 * g.T;
   std::vector<double> F(s0, 0.0), G(s0, 0.0), H(s0, 0.0);
-  if (g.cfg.godunov == 1) {
+  if (g.cfg.godunov == 2) {
+    const int f = ctu_hancock(g, P.data(), U.data(), F.data(), G.data(), H.data(), dt, acc);
+    if (f) { *err = f; return false; }
+  }
+  else if (g.cfg.godunov == 1) {
     // WenoSplit::drive_sweeps_3d, Mara/weno-split.cpp:291-335 (U is the synchronised copy)
     weno_split_sweep(g, U.data(), P.data(), F.data(), 1, acc);
     weno_split_sweep(g, U.data(), P.data(), G.data(), 2, acc);
@@ -399,36 +529,36 @@ bool advance_state(Grid &g, std::vector<double> &U, double dt, Acc &acc, int *er
   std::vector<double> L, U1(n);
   switch (g.cfg.rk) {
   case 1:
-    if (!dUdt(g, U, L, acc, err)) return false;
+    if (!dUdt(g, U, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) U[i] += dt*L[i];
     return true;
   case 2:
-    if (!dUdt(g, U, L, acc, err)) return false;
+    if (!dUdt(g, U, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) U1[i] = U[i] + dt*L[i];
-    if (!dUdt(g, U1, L, acc, err)) return false;
+    if (!dUdt(g, U1, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) U[i] = 0.5*(U[i] + U1[i] + dt*L[i]);
     return true;
   case 4: {
     std::vector<double> L1(n), L2(n), L3(n), T(n);
-    if (!dUdt(g, U, L, acc, err)) return false;
+    if (!dUdt(g, U, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) L1[i] = dt*L[i];
     for (size_t i = 0; i < n; ++i) T[i] = U[i] + 0.5*L1[i];
-    if (!dUdt(g, T, L, acc, err)) return false;
+    if (!dUdt(g, T, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) L2[i] = dt*L[i];
     for (size_t i = 0; i < n; ++i) T[i] = U[i] + 0.5*L2[i];
-    if (!dUdt(g, T, L, acc, err)) return false;
+    if (!dUdt(g, T, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) L3[i] = dt*L[i];
     for (size_t i = 0; i < n; ++i) T[i] = U[i] + 1.0*L3[i];
-    if (!dUdt(g, T, L, acc, err)) return false;
+    if (!dUdt(g, T, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) U[i] += (1.0/6.0)*(L1[i] + 2.0*L2[i] + 2.0*L3[i] + dt*L[i]);
     return true;
   }
   default:
-    if (!dUdt(g, U, L, acc, err)) return false;
+    if (!dUdt(g, U, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) U1[i] = U[i] + dt*L[i];
-    if (!dUdt(g, U1, L, acc, err)) return false;
+    if (!dUdt(g, U1, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) U1[i] = 3./4*U[i] + 1./4*U1[i] + 1./4*dt*L[i];
-    if (!dUdt(g, U1, L, acc, err)) return false;
+    if (!dUdt(g, U1, L, acc, err, dt)) return false;
     for (size_t i = 0; i < n; ++i) U[i] = 1./3*U[i] + 2./3*U1[i] + 2./3*dt*L[i];
     return true;
   }

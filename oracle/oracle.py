@@ -26,7 +26,7 @@ REF_SO = os.path.join(HERE, "_ref", "libmararef.so")
 RECON = {"pcm": 0, "plm": 1, "weno5": 2}
 IS_MODES = {"js96": 0, "b08": 1, "sz10": 2}
 RIEMANN = {"hll": 0, "hllc": 1, "hlld": 2}
-GODUNOV = {"plm-split": 0, "weno-split": 1}
+GODUNOV = {"plm-split": 0, "weno-split": 1, "plm-muscl": 2}
 EIGEN = {"component": 0, "rmhd": 1}
 ADVANCE = {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}
 # weno.h ReconstructOperation

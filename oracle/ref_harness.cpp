@@ -107,7 +107,11 @@ extern "C" {
 void *ref_create(const oracle_config *c)
 {
   ref_start(c->gamma, c->godunov == 1);
-  if (c->godunov == 1) {
+  if (c->godunov == 2) {
+    App->godunov = new PlmCtuHancockOperator;
+    App->godunov->SetPlmTheta(c->theta);        // the operator's own limiter parameter
+  }
+  else if (c->godunov == 1) {
     App->godunov = new WenoSplit;
     GodunovOperator::fluxsplit_method = GodunovOperator::FLUXSPLIT_LOCAL_LAX_FRIEDRICHS;
     static_cast<RmhdWithEigensystem *>(App->fluid)->mode = c->eigen;

@@ -124,6 +124,13 @@ GRID_CASES = [
     # oracle's RMHD eigensystem
     ("weno_split_char_rk3", "smooth", dict(godunov="weno-split", eigen="rmhd", advance="rk3"), 3),
     ("weno_split_char_rk2_jump", "jump", dict(godunov="weno-split", eigen="rmhd", advance="rk2"), 3),
+    # plm-muscl: the reference's PlmCtuHancockOperator (its own theta)
+    ("plm_muscl_hlld_single", "smooth", dict(godunov="plm-muscl", riemann="hlld", advance="single"), 3),
+    ("plm_muscl_hll_rk2_theta125", "smooth", dict(godunov="plm-muscl", riemann="hll", advance="rk2",
+                                                  theta=1.25), 3),
+    ("plm_muscl_hlld_single_jump", "jump", dict(godunov="plm-muscl", riemann="hlld", advance="single"), 3),
+    ("plm_muscl_hlld_superluminal", "superluminal", dict(godunov="plm-muscl", riemann="hlld",
+                                                          advance="single"), 1),
 ]
 
 

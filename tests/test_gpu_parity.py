@@ -163,7 +163,9 @@ GRIDS = ["plm_hlld_rk2", "weno5_hlld_rk3", "plm_hllc_rk2", "pcm_hll_single", "pl
          "weno5_b08_hlld_rk3", "weno5_sz10_hlld_rk3", "plm_hlld_rk2_jump", "plm_theta1_hlld_rk2",
          "plm_hlld_rk2_superluminal", "plm_hlld_rk2_superluminal_ereset",
          "plm_hlld_rk2_cold_pfloor", "weno_split_rk3", "weno_split_rk2_jump", "weno_split_sz10_rk3",
-         "weno_split_rk2_superluminal_ereset", "weno_split_char_rk3", "weno_split_char_rk2_jump"]
+         "weno_split_rk2_superluminal_ereset", "weno_split_char_rk3", "weno_split_char_rk2_jump",
+         "plm_muscl_hlld_single", "plm_muscl_hll_rk2_theta125", "plm_muscl_hlld_single_jump",
+         "plm_muscl_hlld_superluminal"]
 
 
 @pytest.mark.parametrize("name", GRIDS)
