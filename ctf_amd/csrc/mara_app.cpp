@@ -67,6 +67,9 @@ struct App {
 };
 
 App *g_app = nullptr;
+// the CTU operator's limiter is a file-scope static of the reference (Mara/ctu-hancock.cpp:12),
+// shared by every PlmCtuHancockOperator and kept across Mara.start() for the process
+double g_ctu_theta = 2.0;
 std::string g_err;
 
 int fail(const std::string &msg, int code = MARAGPU_E_INVALID)
@@ -136,7 +139,18 @@ int ensure_ctx(bool for_advance = true)
     a->ctx = nullptr;
     a->prim_on_device = false;
   }
-  int rc = maragpu_create(&a->cfg, &a->ctx);
+  maragpu_config cc = a->cfg;
+  bool stand_in = false;
+  if (cc.godunov == MARAGPU_PLM_MUSCL) {
+    cc.theta = g_ctu_theta;
+    if (cc.precision != MARAGPU_FP64 || cc.c2p != MARAGPU_C2P_PLAIN) {
+      if (for_advance) return fail("[Mara] plm-muscl runs in precision 'fp64' with the 'plain' c2p");
+      // a measure needs no operator: a plm-split context serves it, rebuilt for the next advance
+      cc.godunov = MARAGPU_PLM_SPLIT;
+      stand_in = true;
+    }
+  }
+  int rc = maragpu_create(&cc, &a->ctx);
   if (rc) return fail(std::string("[Mara] device context: ") + maragpu_strerror(rc), rc);
   if (!a->sampled.empty()) {
     maragpu_ou_field *f = nullptr;
@@ -149,7 +163,7 @@ int ensure_ctx(bool for_advance = true)
     if (!a->mlp && (rc = load_mlp(a))) return rc;
     maragpu_attach_mlp(a->ctx, a->mlp);
   }
-  a->dirty = false;
+  a->dirty = stand_in;
   return 0;
 }
 
@@ -258,9 +272,12 @@ int mara_set_godunov(const char *key)
     mark_dirty();
     return 0;
   }
-  if (key && std::strcmp(key, "plm-muscl") == 0)
-    return fail(std::string("[Mara] integration scheme '") + key +
-                "' is outside this build (plm-split, weno-split)");
+  if (key && std::strcmp(key, "plm-muscl") == 0) {
+    app()->have_godunov = true;
+    app()->cfg.godunov = MARAGPU_PLM_MUSCL;
+    mark_dirty();
+    return 0;
+  }
   return fail(std::string("[Mara] no such integration scheme '") + (key ? key : "(null)") + "'");
 }
 
@@ -347,7 +364,12 @@ int mara_config_solver_num(const char *key, double value)
   if (int rc = need_app()) return rc;
   if (!key) return fail("[Mara] config_solver: null key");
   maragpu_config &c = app()->cfg;
-  if (!std::strcmp(key, "theta")) c.theta = value;
+  if (!std::strcmp(key, "theta")) {
+    c.theta = value;
+    // luaC_config_solver also calls godunov->SetPlmTheta (Mara/mara.cpp:1068-1075), which
+    // only the CTU operator implements: its limiter follows only while it is the operator
+    if (app()->have_godunov && c.godunov == MARAGPU_PLM_MUSCL) g_ctu_theta = value;
+  }
   else if (!std::strcmp(key, "sz10A")) c.sz10A = value;
   else if (!std::strcmp(key, "pfloor")) c.pfloor = value;
   else if (!std::strcmp(key, "preset")) c.preset = value;
@@ -462,6 +484,18 @@ int mara_measure_mean_max_divB(double *mean, double *max)
   if (int rc = measure_ready()) return rc;
   const int rc = maragpu_measure_divB(app()->ctx, mean, max, nullptr);
   return rc ? fail(std::string("[Mara] measure: ") + maragpu_strerror(rc), rc) : 0;
+}
+
+int mara_set_plm_theta(double theta)
+{
+  if (int rc = need_app()) return rc;
+  // luaC_set_plm_theta (Mara/mara.cpp:985-993): only the CTU operator takes it
+  if (!app()->have_godunov) return fail("[Mara] need a godunov operator for this");
+  if (app()->cfg.godunov == MARAGPU_PLM_MUSCL) {
+    g_ctu_theta = theta;
+    mark_dirty();
+  }
+  return 0;
 }
 
 int mara_set_units(double a, double b, double c)

@@ -70,6 +70,7 @@ struct DevStats {
   unsigned long long stage_failed[6];  // failed zones per c2p pass (ghost images included)
   unsigned long long stage_reset[6];   // zones reset by the emergency repair per pass (weno-split)
   unsigned long long eig_identity;     // weno-split interfaces whose eigensystem was rejected
+  unsigned long long ctu_failed[4][4]; // plm-muscl: failed checks per stage and abort point
 };
 
 /* Interfaces of an HLLD sweep handed to the full FP64 solver (k_flux_redo) before the
@@ -914,6 +915,175 @@ __global__ __launch_bounds__(kBlock) void k_flux_weno_split_char(const double *_
 }
 
 /* --------------------------------------------------------------------------
+ * plm-muscl: PlmCtuHancockOperator::ctu_hancock (Mara/ctu-hancock.cpp:71-494), 3-D, FP64.
+ * The operator's fluxes F, G, H (after CT, k_update) are those of the reference: the step-2
+ * Godunov fluxes outside [sx, T - sx) and the corrector fluxes inside. Failed checks are
+ * counted per abort point (ctu_failed[stage][point]); the step reports the first nonzero
+ * count, as the reference throws at the first abort point with failures.
+ * ------------------------------------------------------------------------ */
+/* the operator's own limited slope (plm_minmod with its theta, :22-31) of field q along S at
+ * zone z, zero outside the flat range [sx, T - sx) where the reference computes slopes */
+__device__ __forceinline__ double ctu_slope(const double *__restrict__ P, long T, long sxz, long z, long S, int q,
+                                            double theta)
+{
+#pragma clang fp contract(off)
+  if (z < sxz || z >= T - sxz) return 0.0;
+  const double ul = P[q * T + z - S], u0 = P[q * T + z], ur = P[q * T + z + S];
+  const double a = theta * (u0 - ul);
+  const double b = 0.5 * (ur - ul);
+  const double c = theta * (ur - u0);
+  const double sa = (double)((a > 0) - (a < 0)), sb = (double)((b > 0) - (b < 0)), sc = (double)((c > 0) - (c < 0));
+  const double ab = fabs(a) < fabs(b) ? fabs(a) : fabs(b);
+  const double m = ab < fabs(c) ? ab : fabs(c);
+  return 0.25 * fabs(sa + sb) * (sa + sc) * m;
+}
+
+/* Godunov fluxes along DIM of the half-slope states over faces z | z+S, z in [begin, T - sx):
+ * from P with its own slopes (step 2, begin 0) or from a predictor Ps with P's slopes
+ * (step 4, begin sx). An unphysical state counts a failure and leaves the flux unwritten. */
+template <int DIM>
+__global__ __launch_bounds__(kBlock, 2) void k_ctu_faces(const double *__restrict__ P, const double *__restrict__ Ps,
+                                                         double *__restrict__ F, GridDesc g, PhysParams pp, int solver,
+                                                         long begin, DevStats *st, unsigned long long *fail)
+{
+#pragma clang fp contract(off)
+  const long T = g.T, sxz = g.sx;
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  const long z = begin + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double lam_out = 0.0;
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  if (z < T - sxz) {
+    double pl[8], pr[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      pr[q] = Ps[q * T + z + S] - 0.5 * ctu_slope(P, T, sxz, z + S, S, q, pp.theta);
+      pl[q] = Ps[q * T + z] + 0.5 * ctu_slope(P, T, sxz, z, S, q, pp.theta);
+    }
+    if (check_prim(pl) || check_prim(pr)) cnt[0] = 1;
+    else {
+      double f[8], lam = 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) f[q] = F[q * T + z];
+      FluxCounters fc = {0, 0, 0, 0};
+      riemann_dispatch<double, DIM>(solver, pl, pr, f, pp.gamma, pp.pfloor, lam, fc);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) F[q * T + z] = f[q];
+      lam_out = lam;
+      cnt[1] = (unsigned long long)fc.hlld_ok;
+      cnt[2] = (unsigned long long)fc.hlld_fail;
+      cnt[3] = (unsigned long long)fc.c2p_iters;
+    }
+  }
+  block_max_lambda(lam_out, &st->lam_bits);
+  unsigned long long *const dst[4] = { fail, &st->hlld_success, &st->hlld_failure, &st->hlld_c2p_iters };
+  block_add<4>(cnt, dst);
+}
+
+/* the physical flux along DIM of the state P (U from it) */
+template <int DIM>
+__device__ __forceinline__ void ctu_side_flux(const double *Ps, double *F, double gamma)
+{
+  double u[8], a, b;
+  prim_to_cons(Ps, u, gamma - 1.0);
+  flux_and_eigenvalues<double, DIM, false>(u, Ps, F, a, b, gamma);
+}
+
+/* Step 3: per zone z in [sx, T), the states on the inner faces P -+ slope/2 of each axis, their
+ * fluxes, and the Hancock predictor of each axis with the transverse step-2 fluxes */
+__global__ __launch_bounds__(kBlock, 2) void k_ctu_predict(const double *__restrict__ P, const double *__restrict__ U,
+                                                           const double *__restrict__ F, const double *__restrict__ G,
+                                                           const double *__restrict__ H, double *__restrict__ Ux,
+                                                           double *__restrict__ Uy, double *__restrict__ Uz, GridDesc g,
+                                                           PhysParams pp, double dt, unsigned long long *fail)
+{
+#pragma clang fp contract(off)
+  const long T = g.T, sxz = g.sx, sy = g.sy, sz = g.sz;
+  const long z = sxz + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long cnt[1] = {0};
+  if (z < T) {
+    double FL[8], FR[8], GL[8], GR[8], HL[8], HR[8];
+    const long Sd[3] = {g.sx, g.sy, g.sz};
+    double *Fo[3][2] = {{FL, FR}, {GL, GR}, {HL, HR}};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      double PL[8], PR[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double s = ctu_slope(P, T, sxz, z, Sd[d], q, pp.theta);
+        PL[q] = P[q * T + z] - 0.5 * s;
+        PR[q] = P[q * T + z] + 0.5 * s;
+      }
+      if (check_prim(PL) || check_prim(PR)) { cnt[0] += 1; continue; }
+      if (d == 0) { ctu_side_flux<1>(PL, Fo[0][0], pp.gamma); ctu_side_flux<1>(PR, Fo[0][1], pp.gamma); }
+      if (d == 1) { ctu_side_flux<2>(PL, Fo[1][0], pp.gamma); ctu_side_flux<2>(PR, Fo[1][1], pp.gamma); }
+      if (d == 2) { ctu_side_flux<3>(PL, Fo[2][0], pp.gamma); ctu_side_flux<3>(PR, Fo[2][1], pp.gamma); }
+    }
+    if (cnt[0] == 0) {
+      const double dx = g.dx, dy = g.dy, dz = g.dz;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const long m = q * T + z;
+        const double dF = (F[m] - F[m - sxz]) / dx, dG = (G[m] - G[m - sy]) / dy, dH = (H[m] - H[m - sz]) / dz;
+        Ux[m] = U[m] - ((FR[q] - FL[q]) / dx + dG + dH) * 0.5 * dt;
+        Uy[m] = U[m] - ((GR[q] - GL[q]) / dy + dH + dF) * 0.5 * dt;
+        Uz[m] = U[m] - ((HR[q] - HL[q]) / dz + dF + dG) * 0.5 * dt;
+      }
+    }
+  }
+  unsigned long long *const dst[1] = { fail };
+  block_add<1>(cnt, dst);
+}
+
+/* the predictors' ghost zones from their periodic images (PeriodicBoundary::ApplyBoundaries) */
+__global__ __launch_bounds__(kBlock) void k_ctu_sync(double *__restrict__ Ux, double *__restrict__ Uy,
+                                                     double *__restrict__ Uz, GridDesc g)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  if (t >= ni) return;
+  const long z = interior_zone(g, t);
+  const int k = (int)(z % g.n[2]), j = (int)((z / g.n[2]) % g.n[1]), i = (int)(z / ((long)g.n[1] * g.n[2]));
+  double *A[3] = {Ux, Uy, Uz};
+  for (int d = 0; d < 3; ++d) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = A[d][q * g.T + z];
+    scatter_images(g, A[d], g.T, i, j, k, v);
+  }
+}
+
+/* the predictors inverted (Rmhd::ConsToPrim) from P at every zone, ghosts included; a
+ * failure keeps P there and counts once per axis */
+__global__ __launch_bounds__(kBlock, 3) void k_ctu_c2p(const double *__restrict__ P, const double *__restrict__ Ux,
+                                                       const double *__restrict__ Uy, const double *__restrict__ Uz,
+                                                       double *__restrict__ Px, double *__restrict__ Py,
+                                                       double *__restrict__ Pz, GridDesc g, double gamma, double pfloor,
+                                                       DevStats *st, unsigned long long *fail)
+{
+  const long z = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long T = g.T;
+  unsigned long long cnt[3] = {0, 0, 0};
+  if (z < T) {
+    const double *Ud[3] = {Ux, Uy, Uz};
+    double *Pd[3] = {Px, Py, Pz};
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d) {
+      double u[8], p[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) { u[q] = Ud[d][q * T + z]; p[q] = P[q * T + z]; }
+      int iters = 0, which = 0;
+      if (cons_to_prim(u, p, gamma, pfloor, true, iters, which) != 0) cnt[0] += 1;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Pd[d][q * T + z] = p[q];
+      cnt[1] += 1;
+      cnt[2] += (unsigned long long)iters;
+    }
+  }
+  unsigned long long *const dst[3] = { fail, &st->c2p_zones, &st->c2p_iters };
+  block_add<3>(cnt, dst);
+}
+
+/* --------------------------------------------------------------------------
  * CT + divergence + RK stage (FP64, no contraction: the same roundings as the reference)
  * ------------------------------------------------------------------------ */
 enum StageKind {
@@ -1546,6 +1716,8 @@ struct maragpu_ctx {
   void *FZ = nullptr;
   // the measure table: per-block partials and the reduced values
   double *dmeas = nullptr;
+  // plm-muscl: the three predicted conserved states and their primitives (8 planes each)
+  double *ctuU = nullptr, *ctuP = nullptr;
   // driving (DrivingProcedure): the field sampled at every zone, 3 planes of T doubles
   double *Fdrive = nullptr;
   // cooling (CoolingModuleT4 / E4): kind 0 none, 1 T4, 2 E4; per-block partials of the
@@ -1670,7 +1842,11 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (cfg->rk < 1 || cfg->rk > 4 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->recon < 0 ||
       cfg->recon > 2 || cfg->precision < 0 || cfg->precision > 1 || !(cfg->gamma > 1.0))
     return MARAGPU_E_INVALID;
-  if (cfg->godunov != MARAGPU_PLM_SPLIT && cfg->godunov != MARAGPU_WENO_SPLIT) return MARAGPU_E_INVALID;
+  if (cfg->godunov != MARAGPU_PLM_SPLIT && cfg->godunov != MARAGPU_WENO_SPLIT && cfg->godunov != MARAGPU_PLM_MUSCL)
+    return MARAGPU_E_INVALID;
+  // plm-muscl runs the reference's FP64 operator with the plain c2p
+  if (cfg->godunov == MARAGPU_PLM_MUSCL && (cfg->precision != MARAGPU_FP64 || cfg->c2p != MARAGPU_C2P_PLAIN))
+    return MARAGPU_E_INVALID;
   if (cfg->eigen != MARAGPU_EIG_COMPONENT && cfg->eigen != MARAGPU_EIG_RMHD) return MARAGPU_E_INVALID;
   if (cfg->godunov == MARAGPU_WENO_SPLIT && cfg->Ng < 3) return MARAGPU_E_INVALID;
   int ndev = 0;
@@ -1703,7 +1879,13 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
         hipMalloc(&c->FZ, (size_t)9 * c->g.T * rb) != hipSuccess)
       return fail_free(c);
   }
-  if (cfg->riemann == MARAGPU_HLLD && cfg->godunov != MARAGPU_WENO_SPLIT) {
+  if (cfg->godunov == MARAGPU_PLM_MUSCL) {
+    if (hipMalloc(&c->Uw, nd * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->ctuU, 3 * nd * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->ctuP, 3 * nd * sizeof(double)) != hipSuccess)
+      return fail_free(c);
+  }
+  if (cfg->riemann == MARAGPU_HLLD && cfg->godunov == MARAGPU_PLM_SPLIT) {
     // a slot for every interface of a sweep (4 bytes a slot: half a state plane)
     const long cap = c->g.T;
     c->redo.cap = (unsigned int)cap;
@@ -1739,6 +1921,8 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->Uw);
   hipFree(c->FZ);
   hipFree(c->dmeas);
+  hipFree(c->ctuU);
+  hipFree(c->ctuP);
   hipFree(c->Fdrive);
   hipFree(c->dcool);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
@@ -1832,7 +2016,7 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     for (int d = 1; d <= 3; ++d) {
       EventPair p = mark(ef);
       hipEventRecord(c->ev[p.a], c->stream);
-      if (c->Uw) {
+      if (c->cfg.godunov == MARAGPU_WENO_SPLIT) {
         if (mixed) {
           if (d == 1) launch_weno_split<float, 1>(c, pp);
           if (d == 2) launch_weno_split<float, 2>(c, pp);
@@ -1876,11 +2060,41 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   case 4: kinds = { ST_RK4_1, ST_RK4_2, ST_RK4_3, ST_RK4_4 }; break;
   default: kinds = { ST_EULER, ST_RK3_2, ST_RK3_3 }; break;
   }
+  const bool ctu = c->cfg.godunov == MARAGPU_PLM_MUSCL;
+  auto ctu_fluxes = [&](int s) {
+    // PlmCtuHancockOperator::ctu_hancock on the stage primitives Pw and the synchronised U (Uw)
+    EventPair p = mark(ef);
+    hipEventRecord(c->ev[p.a], c->stream);
+    const long T = g.T, sxz = g.sx;
+    const size_t nd = 8 * (size_t)T;
+    double *F0 = static_cast<double *>(c->F[0]), *F1 = static_cast<double *>(c->F[1]),
+           *F2 = static_cast<double *>(c->F[2]);
+    double *Ux = c->ctuU, *Uy = c->ctuU + nd, *Uz = c->ctuU + 2 * nd;
+    double *Px = c->ctuP, *Py = c->ctuP + nd, *Pz = c->ctuP + 2 * nd;
+    unsigned long long *fl = c->dstats->ctu_failed[s];
+    const int rs = c->cfg.riemann;
+    const long nf = T - sxz;
+    k_ctu_faces<1><<<nblocks(nf), kBlock, 0, c->stream>>>(c->Pw, c->Pw, F0, g, pp, rs, 0, c->dstats, fl + 0);
+    k_ctu_faces<2><<<nblocks(nf), kBlock, 0, c->stream>>>(c->Pw, c->Pw, F1, g, pp, rs, 0, c->dstats, fl + 0);
+    k_ctu_faces<3><<<nblocks(nf), kBlock, 0, c->stream>>>(c->Pw, c->Pw, F2, g, pp, rs, 0, c->dstats, fl + 0);
+    k_ctu_predict<<<nblocks(T - sxz), kBlock, 0, c->stream>>>(c->Pw, c->Uw, F0, F1, F2, Ux, Uy, Uz, g, pp, dt, fl + 1);
+    k_ctu_sync<<<nblocks(ni), kBlock, 0, c->stream>>>(Ux, Uy, Uz, g);
+    k_ctu_c2p<<<nblocks(T), kBlock, 0, c->stream>>>(c->Pw, Ux, Uy, Uz, Px, Py, Pz, g, c->cfg.gamma, c->cfg.pfloor,
+                                                    c->dstats, fl + 2);
+    const long n4 = T - 2 * sxz;
+    if (n4 > 0) {
+      k_ctu_faces<1><<<nblocks(n4), kBlock, 0, c->stream>>>(c->Pw, Px, F0, g, pp, rs, sxz, c->dstats, fl + 3);
+      k_ctu_faces<2><<<nblocks(n4), kBlock, 0, c->stream>>>(c->Pw, Py, F1, g, pp, rs, sxz, c->dstats, fl + 3);
+      k_ctu_faces<3><<<nblocks(n4), kBlock, 0, c->stream>>>(c->Pw, Pz, F2, g, pp, rs, sxz, c->dstats, fl + 3);
+    }
+    hipEventRecord(c->ev[p.b], c->stream);
+  };
   int pass = 0;
   for (size_t s = 0; s < kinds.size(); ++s) {
     int rc = c2p(s == 0, pass++);
     if (rc) return rc;
-    sweeps();
+    if (ctu) ctu_fluxes((int)s);
+    else sweeps();
     update(kinds[s]);
   }
   int rc = c2p(0, pass++);
@@ -1893,8 +2107,17 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
 
   const DevStats &h = *c->hstats;
   int errors = 0;
-  for (int p = 0; p < pass; ++p) {
-    if (h.stage_failed[p]) { errors = (int)h.stage_failed[p]; break; }
+  if (ctu) {
+    // PlmCtuHancockOperator::dUdt ignores the stage c2p count (Mara/ctu-hancock.cpp:52-69):
+    // the first abort point with failures, else the final c2p
+    for (size_t s = 0; s < kinds.size() && !errors; ++s)
+      for (int a = 0; a < 4 && !errors; ++a) errors = (int)h.ctu_failed[s][a];
+    if (!errors) errors = (int)h.stage_failed[pass - 1];
+  }
+  else {
+    for (int p = 0; p < pass; ++p) {
+      if (h.stage_failed[p]) { errors = (int)h.stage_failed[p]; break; }
+    }
   }
   double lam;
   unsigned long long bits = h.lam_bits;

@@ -161,6 +161,9 @@ class _Mara:
     def set_godunov(self, key):
         _chk(native.lib().mara_set_godunov(_s(key)))
 
+    def set_plm_theta(self, theta):
+        _chk(native.lib().mara_set_plm_theta(float(theta)))
+
     def set_advance(self, key):
         _chk(native.lib().mara_set_advance(_s(key)))
 

@@ -25,7 +25,7 @@ IS_MODES = {"js96": JS96, "b08": B08, "sz10": SZ10}
 RIEMANN = {"hll": HLL, "hllc": HLLC, "hlld": HLLD}
 ADVANCE = {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}
 PRECISION = {"fp64": FP64, "mixed": MIXED}
-GODUNOV = {"plm-split": 0, "weno-split": 1}
+GODUNOV = {"plm-split": 0, "weno-split": 1, "plm-muscl": 2}
 EIGEN = {"component": 0, "rmhd": 1}
 C2P = {"plain": C2P_PLAIN, "learned": C2P_LEARNED}
 
@@ -35,7 +35,7 @@ EXPORTS = [
     "maragpu_strerror", "maragpu_batch_prim_to_cons", "maragpu_batch_cons_to_prim",
     "maragpu_batch_flux_eigen", "maragpu_batch_reconstruct", "maragpu_batch_riemann",
     "maragpu_batch_eigensystem", "mara_start", "mara_close", "mara_set_fluid", "mara_set_eos", "mara_set_boundary",
-    "mara_set_riemann", "mara_set_godunov", "mara_set_advance", "mara_config_solver_str",
+    "mara_set_riemann", "mara_set_godunov", "mara_set_plm_theta", "mara_set_advance", "mara_config_solver_str",
     "mara_config_solver_num", "mara_config_solver_bool", "mara_set_domain", "mara_advance",
     "mara_get_timestep", "mara_set_primitive", "mara_measure_mean_max_divB", "mara_last_error",
     "maragpu_measure", "mara_set_units", "mara_measure_mean_velocity", "mara_measure_mean_cons",
@@ -153,6 +153,7 @@ def lib():
         "mara_set_boundary": (C.c_int, [C.c_char_p]),
         "mara_set_riemann": (C.c_int, [C.c_char_p]),
         "mara_set_godunov": (C.c_int, [C.c_char_p]),
+        "mara_set_plm_theta": (C.c_int, [C.c_double]),
         "mara_set_advance": (C.c_int, [C.c_char_p]),
         "mara_config_solver_str": (C.c_int, [C.c_char_p, C.c_char_p]),
         "mara_config_solver_num": (C.c_int, [C.c_char_p, C.c_double]),

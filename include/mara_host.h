@@ -12,6 +12,8 @@
  *   mara_set_riemann                 luaC_set_riemann                (:934-958)   'hll' 'hllc' 'hlld'
  *   mara_set_godunov                 luaC_set_godunov                (:960-984)   'plm-split'
  *                                                                                 'weno-split'
+ *                                                                                 'plm-muscl'
+ *   mara_set_plm_theta               luaC_set_plm_theta              (:985-993)
  *   mara_set_advance                 luaC_set_advance                (:1140-1167) 'single' 'rk2' 'rk3' 'rk4'
  *   mara_config_solver_{str,num,bool} one key of luaC_config_solver  (:996-1138); extra keys
  *                                    'precision' = 'fp64' | 'mixed', 'c2p' = 'plain' | 'learned',
@@ -66,6 +68,9 @@ int mara_measure_mean_max_divB(double *mean, double *max);
  * Time) although the Lua arguments are read as (Mass, Length, Time) (Mara/mara.cpp:871-877,
  * Mara/units.cpp:3-6): a sets the length unit, b the mass unit. Kept as is. */
 int mara_set_units(double a, double b, double c);
+/* the CTU operator's limiter parameter; like config_solver's theta it reaches the operator
+ * only while plm-muscl is selected (the other operators ignore SetPlmTheta) */
+int mara_set_plm_theta(double theta);
 /* The measures, mean_max_divB included, read the array of the last set_primitive
  * (Mara->PrimitiveArray, which advance does not change), uploaded to the device on demand;
  * they need a domain and the fluid, not the rest of the solver. */

@@ -49,7 +49,7 @@ enum { MARAGPU_JS96 = 0, MARAGPU_B08 = 1, MARAGPU_SZ10 = 2 };            /* conf
 enum { MARAGPU_HLL = 0, MARAGPU_HLLC = 1, MARAGPU_HLLD = 2 };            /* set_riemann */
 enum { MARAGPU_FP64 = 0, MARAGPU_MIXED = 1 };                            /* precision */
 enum { MARAGPU_C2P_PLAIN = 0, MARAGPU_C2P_LEARNED = 1 };                 /* c2p initial guess */
-enum { MARAGPU_PLM_SPLIT = 0, MARAGPU_WENO_SPLIT = 1 };                  /* set_godunov */
+enum { MARAGPU_PLM_SPLIT = 0, MARAGPU_WENO_SPLIT = 1, MARAGPU_PLM_MUSCL = 2 };  /* set_godunov */
 enum { MARAGPU_EIG_COMPONENT = 0, MARAGPU_EIG_RMHD = 1 };                /* weno-split projection */
 
 enum {
@@ -77,7 +77,9 @@ typedef struct maragpu_config {
   double pfloor;     /* pressure floor, disabled if <= 0 */
   int ereset;        /* emergency reset of failed zones */
   double preset, dreset;
-  int godunov;       /* MARAGPU_PLM_SPLIT or MARAGPU_WENO_SPLIT (local Lax-Friedrichs splitting) */
+  int godunov;       /* MARAGPU_PLM_SPLIT, MARAGPU_WENO_SPLIT (local Lax-Friedrichs splitting) or
+                        MARAGPU_PLM_MUSCL (CTU-Hancock, FP64 and the plain c2p; theta is then the
+                        operator's own limiter parameter, PlmCtuHancockOperator::SetPlmTheta) */
   int eigen;         /* weno-split projection: MARAGPU_EIG_COMPONENT or MARAGPU_EIG_RMHD */
 } maragpu_config;
 

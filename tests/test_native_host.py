@@ -97,8 +97,12 @@ def test_lua_verbs_reject_like_mara():
         Mara.config_solver({"extrap": "ppm"}, True)
     with pytest.raises(MaraError, match="no such IS"):
         Mara.config_solver({"IS": "xyz"}, True)
-    with pytest.raises(MaraError, match="outside this build"):
-        Mara.set_godunov("plm-muscl")
+    with pytest.raises(MaraError, match="need a godunov operator"):
+        Mara.set_plm_theta(1.5)
+    with pytest.raises(MaraError, match="no such integration scheme"):
+        Mara.set_godunov("plm-ctu")
+    Mara.set_godunov("plm-muscl")
+    Mara.set_plm_theta(1.5)
     with pytest.raises(MaraError, match="five fields"):
         Mara.config_solver({"fsplit": "marq"}, True)
     Mara.set_godunov("weno-split")
