@@ -258,6 +258,30 @@ def weno_split_record(w, N, steps):
     return rec
 
 
+def plm_muscl_record(w, N, steps):
+    """The plm-muscl operator (CTU-Hancock: predictor face fluxes, a Hancock half step per
+    axis, corrector fluxes) against plm-split with the same PLM reconstruction, HLLD and
+    Runge-Kutta scheme, on the same turbulence at N^3, FP64: `steps` timed steps after the
+    priming step and one warm-up step, with the same dt sequence for both."""
+    base = workloads.replace(w.resized(N), extrap="plm", c2p="plain", precision="fp64")
+    P0 = native.init_turbulence(**base.ic_kwargs())
+    rec = {"grid": N, "steps": steps, "riemann": base.riemann, "advance": base.advance,
+           "theta": 2.0}
+    log(f"[bench] plm-split (plm) {N}^3")
+    split = run_form(workloads.replace(base, godunov="plm-split"), P0, steps, 1, total_steps=1 + steps)
+    log(f"[bench] plm-muscl {N}^3")
+    ctu = run_form(workloads.replace(base, godunov="plm-muscl"), P0, steps, 1, dts=split["dts"],
+                   total_steps=1 + steps)
+    for name, r in (("plm_split_plm", split), ("plm_muscl", ctu)):
+        st = r["stats"]
+        rec[name] = {"zone_updates_per_s": r["zone_updates_per_s"], "ms_per_step": r["ms_per_step"],
+                     "flux_ms_per_step": r["flux_ms_per_step"], "c2p_ms_per_step": r["c2p_ms_per_step"],
+                     "update_ms_per_step": r["update_ms_per_step"], "max_divB": r["max_divB"],
+                     "riemann_solves": st.get("hlld_success", 0) + st.get("hlld_failure", 0)}
+    rec["plm_muscl"]["L1_vs_plm_split"] = l1_vs(ctu["P"], split["P"], w.Ng)
+    return rec
+
+
 def traffic_record(w):
     path = os.path.join(ROOT, "profiles", f"traffic_{w.name}.json")
     if os.path.exists(path):
@@ -281,6 +305,9 @@ def main():
                     help="grid of the weno-split measurement (0: skip it)")
     ap.add_argument("--weno-split-steps", type=int, default=3)
     ap.add_argument("--no-driving", action="store_true", help="skip the driving/cooling record")
+    ap.add_argument("--plm-muscl-grid", type=int, default=256,
+                    help="grid of the plm-muscl (CTU-Hancock) measurement (0: skip it)")
+    ap.add_argument("--plm-muscl-steps", type=int, default=3)
     ap.add_argument("--model", default=os.path.join(ROOT, "ctf_amd", "models", "c2p_mlp.bin"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (calibrates the "
@@ -372,6 +399,9 @@ def main():
     ws = None
     if args.weno_split_grid:
         ws = weno_split_record(w, args.weno_split_grid, args.weno_split_steps)
+    pm = None
+    if args.plm_muscl_grid:
+        pm = plm_muscl_record(w, args.plm_muscl_grid, args.plm_muscl_steps)
     dr = None
     if not args.no_driving:
         log("[bench] driving and cooling")
@@ -400,6 +430,8 @@ def main():
     }
     if ws:
         out["weno_split"] = ws
+    if pm:
+        out["plm_muscl"] = pm
     if dr:
         out["driving"] = dr
     if not args.no_cpu_baseline:

@@ -938,32 +938,43 @@ __device__ __forceinline__ double ctu_slope(const double *__restrict__ P, long T
   return 0.25 * fabs(sa + sb) * (sa + sc) * m;
 }
 
-/* Godunov fluxes along DIM of the half-slope states over faces z | z+S, z in [begin, T - sx):
- * from P with its own slopes (step 2, begin 0) or from a predictor Ps with P's slopes
- * (step 4, begin sx). An unphysical state counts a failure and leaves the flux unwritten. */
+/* the half-slope states of face z | z+S along S: from P with its own slopes (step 2) or
+ * from a predictor Ps with P's slopes (step 4) */
+__device__ __forceinline__ void ctu_face_states(const double *__restrict__ P, const double *__restrict__ Ps, long T,
+                                                long sxz, long S, long z, double theta, double *pl, double *pr)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    pr[q] = Ps[q * T + z + S] - 0.5 * ctu_slope(P, T, sxz, z + S, S, q, theta);
+    pl[q] = Ps[q * T + z] + 0.5 * ctu_slope(P, T, sxz, z, S, q, theta);
+  }
+}
+
+/* Godunov fluxes along DIM of the half-slope states over faces z | z+S, z in [begin, T - sx),
+ * with the full solver (HLL, HLLC; HLLD through the redo). An unphysical state counts a
+ * failure and leaves the flux unwritten. A solver error also leaves it: then the flux is
+ * what the reference's array holds there, zero in step 2 (fresh arrays) and the step-2
+ * flux in step 4 (`keep`). */
 template <int DIM>
 __global__ __launch_bounds__(kBlock, 2) void k_ctu_faces(const double *__restrict__ P, const double *__restrict__ Ps,
                                                          double *__restrict__ F, GridDesc g, PhysParams pp, int solver,
-                                                         long begin, DevStats *st, unsigned long long *fail)
+                                                         long begin, int keep, DevStats *st,
+                                                         unsigned long long *fail)
 {
-#pragma clang fp contract(off)
   const long T = g.T, sxz = g.sx;
   const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
   const long z = begin + (long)blockIdx.x * blockDim.x + threadIdx.x;
   double lam_out = 0.0;
-  unsigned long long cnt[4] = {0, 0, 0, 0};
+  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
   if (z < T - sxz) {
     double pl[8], pr[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      pr[q] = Ps[q * T + z + S] - 0.5 * ctu_slope(P, T, sxz, z + S, S, q, pp.theta);
-      pl[q] = Ps[q * T + z] + 0.5 * ctu_slope(P, T, sxz, z, S, q, pp.theta);
-    }
+    ctu_face_states(P, Ps, T, sxz, S, z, pp.theta, pl, pr);
     if (check_prim(pl) || check_prim(pr)) cnt[0] = 1;
     else {
       double f[8], lam = 0.0;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) f[q] = F[q * T + z];
+      for (int q = 0; q < 8; ++q) f[q] = keep ? F[q * T + z] : 0.0;
       FluxCounters fc = {0, 0, 0, 0};
       riemann_dispatch<double, DIM>(solver, pl, pr, f, pp.gamma, pp.pfloor, lam, fc);
 #pragma unroll
@@ -972,11 +983,88 @@ __global__ __launch_bounds__(kBlock, 2) void k_ctu_faces(const double *__restric
       cnt[1] = (unsigned long long)fc.hlld_ok;
       cnt[2] = (unsigned long long)fc.hlld_fail;
       cnt[3] = (unsigned long long)fc.c2p_iters;
+      cnt[4] = (unsigned long long)fc.newton_iters;
     }
   }
   block_max_lambda(lam_out, &st->lam_bits);
-  unsigned long long *const dst[4] = { fail, &st->hlld_success, &st->hlld_failure, &st->hlld_c2p_iters };
+  unsigned long long *const dst[5] = { fail, &st->hlld_success, &st->hlld_failure, &st->hlld_c2p_iters,
+                                       &st->hlld_newton_iters };
+  block_add<5>(cnt, dst);
+}
+
+/* The HLLD faces of k_ctu_faces on the common path (riemann_hlld in FAST mode, a lower
+ * register peak): a face that needs the c2p cascade, a later Newton attempt, the small
+ * normal field branch or the HLLC fallback is queued for k_ctu_faces_redo. The queue holds
+ * every face, so it cannot overflow. */
+template <int DIM>
+__global__ __launch_bounds__(kBlock, kFastFluxWaves64)
+void k_ctu_faces_fast(const double *__restrict__ P, const double *__restrict__ Ps, double *__restrict__ F,
+                      GridDesc g, PhysParams pp, long begin, DevStats *st, unsigned long long *fail, RedoQueue rq)
+{
+  const long T = g.T, sxz = g.sx;
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  const long z = begin + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double lam_out = 0.0;
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  bool defer = false;
+  if (z < T - sxz) {
+    double pl[8], pr[8];
+    ctu_face_states(P, Ps, T, sxz, S, z, pp.theta, pl, pr);
+    if (check_prim(pl) || check_prim(pr)) cnt[0] = 1;
+    else {
+      double f[8], lam = 0.0;
+      FluxCounters fc = {0, 0, 0, 0};
+      defer = riemann_hlld<double, DIM, true>(pl, pr, f, pp.gamma, pp.pfloor, lam, fc) != 0;
+      if (!defer) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) F[q * T + z] = f[q];
+        cnt[1] = 1;
+        cnt[2] = (unsigned long long)fc.c2p_iters;
+        cnt[3] = (unsigned long long)fc.newton_iters;
+      }
+      lam_out = lam;
+    }
+  }
+  queue_push(rq, defer, z);
+  block_max_lambda(lam_out, &st->lam_bits);
+  unsigned long long *const dst[4] = { fail, &st->hlld_success, &st->hlld_c2p_iters, &st->hlld_newton_iters };
   block_add<4>(cnt, dst);
+}
+
+/* The queued faces of k_ctu_faces_fast by the full HLLD method (attempt ladder, HLLC
+ * fallback); a grid-stride loop over the queue count. Their states passed the physical
+ * check in the fast pass. */
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_ctu_faces_redo(const double *__restrict__ P, const double *__restrict__ Ps,
+                                                           double *__restrict__ F, GridDesc g, PhysParams pp, int keep,
+                                                           DevStats *st, RedoQueue rq)
+{
+  const unsigned int n = min(*rq.count, rq.cap);
+  const long T = g.T, sxz = g.sx;
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  double lam_out = 0.0;
+  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const long z = (long)rq.z[k];
+    double pl[8], pr[8], f[8], lam = 0.0;
+    ctu_face_states(P, Ps, T, sxz, S, z, pp.theta, pl, pr);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) f[q] = keep ? F[q * T + z] : 0.0;
+    FluxCounters fc = {0, 0, 0, 0};
+    riemann_hlld<double, DIM>(pl, pr, f, pp.gamma, pp.pfloor, lam, fc);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) F[q * T + z] = f[q];
+    lam_out = fmax(lam_out, lam);
+    cnt[0] += (unsigned long long)fc.hlld_ok;
+    cnt[1] += (unsigned long long)fc.hlld_fail;
+    cnt[2] += (unsigned long long)fc.c2p_iters;
+    cnt[3] += (unsigned long long)fc.newton_iters;
+    cnt[4] += 1;
+  }
+  block_max_lambda(lam_out, &st->lam_bits);
+  unsigned long long *const dst[5] = { &st->hlld_success, &st->hlld_failure, &st->hlld_c2p_iters,
+                                       &st->hlld_newton_iters, &st->redo };
+  block_add<5>(cnt, dst);
 }
 
 /* the physical flux along DIM of the state P (U from it) */
@@ -1885,8 +1973,8 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
         hipMalloc(&c->ctuP, 3 * nd * sizeof(double)) != hipSuccess)
       return fail_free(c);
   }
-  if (cfg->riemann == MARAGPU_HLLD && cfg->godunov == MARAGPU_PLM_SPLIT) {
-    // a slot for every interface of a sweep (4 bytes a slot: half a state plane)
+  if (cfg->riemann == MARAGPU_HLLD && cfg->godunov != MARAGPU_WENO_SPLIT) {
+    // a slot for every interface of a sweep or face pass (4 bytes a slot: half a state plane)
     const long cap = c->g.T;
     c->redo.cap = (unsigned int)cap;
     if (hipMalloc(&c->redo.count, sizeof(unsigned int)) != hipSuccess ||
@@ -2073,20 +2161,36 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     double *Px = c->ctuP, *Py = c->ctuP + nd, *Pz = c->ctuP + 2 * nd;
     unsigned long long *fl = c->dstats->ctu_failed[s];
     const int rs = c->cfg.riemann;
-    const long nf = T - sxz;
-    k_ctu_faces<1><<<nblocks(nf), kBlock, 0, c->stream>>>(c->Pw, c->Pw, F0, g, pp, rs, 0, c->dstats, fl + 0);
-    k_ctu_faces<2><<<nblocks(nf), kBlock, 0, c->stream>>>(c->Pw, c->Pw, F1, g, pp, rs, 0, c->dstats, fl + 0);
-    k_ctu_faces<3><<<nblocks(nf), kBlock, 0, c->stream>>>(c->Pw, c->Pw, F2, g, pp, rs, 0, c->dstats, fl + 0);
+    // the faces of one axis over [begin, T - sx): HLLD on the common path and its queue,
+    // HLL and HLLC by the full kernel
+    auto faces = [&](int dim, const double *Ps, double *F, long begin, int keep, unsigned long long *f) {
+      const long n = T - sxz - begin;
+      if (n <= 0) return;
+      if (c->redo.z) {
+        hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
+        if (dim == 1) k_ctu_faces_fast<1><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
+        if (dim == 2) k_ctu_faces_fast<2><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
+        if (dim == 3) k_ctu_faces_fast<3><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
+        if (dim == 1) k_ctu_faces_redo<1><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, c->redo);
+        if (dim == 2) k_ctu_faces_redo<2><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, c->redo);
+        if (dim == 3) k_ctu_faces_redo<3><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, c->redo);
+      }
+      else {
+        if (dim == 1) k_ctu_faces<1><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, rs, begin, keep, c->dstats, f);
+        if (dim == 2) k_ctu_faces<2><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, rs, begin, keep, c->dstats, f);
+        if (dim == 3) k_ctu_faces<3><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, rs, begin, keep, c->dstats, f);
+      }
+    };
+    faces(1, c->Pw, F0, 0, 0, fl + 0);
+    faces(2, c->Pw, F1, 0, 0, fl + 0);
+    faces(3, c->Pw, F2, 0, 0, fl + 0);
     k_ctu_predict<<<nblocks(T - sxz), kBlock, 0, c->stream>>>(c->Pw, c->Uw, F0, F1, F2, Ux, Uy, Uz, g, pp, dt, fl + 1);
     k_ctu_sync<<<nblocks(ni), kBlock, 0, c->stream>>>(Ux, Uy, Uz, g);
     k_ctu_c2p<<<nblocks(T), kBlock, 0, c->stream>>>(c->Pw, Ux, Uy, Uz, Px, Py, Pz, g, c->cfg.gamma, c->cfg.pfloor,
                                                     c->dstats, fl + 2);
-    const long n4 = T - 2 * sxz;
-    if (n4 > 0) {
-      k_ctu_faces<1><<<nblocks(n4), kBlock, 0, c->stream>>>(c->Pw, Px, F0, g, pp, rs, sxz, c->dstats, fl + 3);
-      k_ctu_faces<2><<<nblocks(n4), kBlock, 0, c->stream>>>(c->Pw, Py, F1, g, pp, rs, sxz, c->dstats, fl + 3);
-      k_ctu_faces<3><<<nblocks(n4), kBlock, 0, c->stream>>>(c->Pw, Pz, F2, g, pp, rs, sxz, c->dstats, fl + 3);
-    }
+    faces(1, Px, F0, sxz, 1, fl + 3);
+    faces(2, Py, F1, sxz, 1, fl + 3);
+    faces(3, Pz, F2, sxz, 1, fl + 3);
     hipEventRecord(c->ev[p.b], c->stream);
   };
   int pass = 0;
