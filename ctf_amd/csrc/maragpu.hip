@@ -196,6 +196,34 @@ __device__ __forceinline__ long interior_zone(const GridDesc &g, long t)
   return ((i + g.Ng) * g.n[1] + (j + g.Ng)) * g.n[2] + (k + g.Ng);
 }
 
+/* one field's interior, (i, j, k) order with k fastest, out of / into its SoA plane: the
+ * hyperslab DataManagerHDF5 moves between Mara's array and a prim/<field> dataset
+ * (modules/unigrid.lua:243-292). Into the plane, every periodic image is written too. */
+__global__ void k_field_out(const double *__restrict__ plane, double *__restrict__ out, GridDesc g)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.N[0] * g.N[1] * g.N[2]) return;
+  out[t] = plane[interior_zone(g, t)];
+}
+
+__global__ void k_field_in(const double *__restrict__ in, double *__restrict__ plane, GridDesc g)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)g.N[0] * g.N[1] * g.N[2]) return;
+  const int k = (int)(t % g.N[2]) + g.Ng;
+  const int j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+  const int i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+  const double v = in[t];
+  int li[3], lj[3], lk[3];
+  int ni = 0, nj = 0, nk = 0;
+  li[ni++] = i; if (i >= g.N[0]) li[ni++] = i - g.N[0]; if (i < 2 * g.Ng) li[ni++] = i + g.N[0];
+  lj[nj++] = j; if (j >= g.N[1]) lj[nj++] = j - g.N[1]; if (j < 2 * g.Ng) lj[nj++] = j + g.N[1];
+  lk[nk++] = k; if (k >= g.N[2]) lk[nk++] = k - g.N[2]; if (k < 2 * g.Ng) lk[nk++] = k + g.N[2];
+  for (int a = 0; a < ni; ++a)
+    for (int b = 0; b < nj; ++b)
+      for (int c = 0; c < nk; ++c) plane[((long)li[a] * g.n[1] + lj[b]) * g.n[2] + lk[c]] = v;
+}
+
 /* GodunovOperator::PrimToCons over the interior (Mara/hydro.cpp:112-123); U0 and U */
 __global__ void k_prim_to_cons(const double *__restrict__ P, double *__restrict__ U0,
                                double *__restrict__ U, GridDesc g, double gamma)
@@ -1915,6 +1943,7 @@ const char *maragpu_strerror(int code)
   case MARAGPU_E_NOMEM: return "device allocation failed";
   case MARAGPU_E_SIZE: return "buffer size does not match the domain";
   case MARAGPU_E_NOMODEL: return "learned c2p requested but no network is loaded";
+  case MARAGPU_E_IO: return "checkpoint file error";
   default: return code > 0 ? "zones failed to recover primitives (state not committed)" : "unknown error";
   }
 }
@@ -2039,6 +2068,41 @@ int maragpu_download_prim(maragpu_ctx *c, double *P, size_t n)
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(P, c->U0, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MARAGPU_OK;
+}
+
+int maragpu_get_config(const maragpu_ctx *c, maragpu_config *cfg)
+{
+  if (!c || !cfg) return MARAGPU_E_INVALID;
+  *cfg = c->cfg;
+  return MARAGPU_OK;
+}
+
+int maragpu_download_field(maragpu_ctx *c, int q, double *out, size_t n)
+{
+  if (!c || !out || q < 0 || q >= 8) return MARAGPU_E_INVALID;
+  const GridDesc &g = c->g;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  if (n != (size_t)ni) return MARAGPU_E_SIZE;
+  // U0 (8 T doubles) is staging outside an advance
+  k_field_out<<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc + (size_t)q * g.T, c->U0, g);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(out, c->U0, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  return MARAGPU_OK;
+}
+
+int maragpu_upload_field(maragpu_ctx *c, int q, const double *in, size_t n)
+{
+  if (!c || !in || q < 0 || q >= 8) return MARAGPU_E_INVALID;
+  const GridDesc &g = c->g;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  if (n != (size_t)ni) return MARAGPU_E_SIZE;
+  HIP_CHECK(hipMemcpyAsync(c->U0, in, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  k_field_in<<<nblocks(ni), kBlock, 0, c->stream>>>(c->U0, c->Pc + (size_t)q * g.T, g);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(c->stream));
+  c->have_state = true;
   return MARAGPU_OK;
 }
 

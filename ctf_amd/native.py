@@ -31,6 +31,7 @@ C2P = {"plain": C2P_PLAIN, "learned": C2P_LEARNED}
 
 EXPORTS = [
     "maragpu_create", "maragpu_destroy", "maragpu_upload_prim", "maragpu_download_prim",
+    "maragpu_get_config", "maragpu_download_field", "maragpu_upload_field",
     "maragpu_advance", "maragpu_get_timestep", "maragpu_measure_divB", "maragpu_kernel_times",
     "maragpu_strerror", "maragpu_batch_prim_to_cons", "maragpu_batch_cons_to_prim",
     "maragpu_batch_flux_eigen", "maragpu_batch_reconstruct", "maragpu_batch_riemann",
@@ -133,6 +134,9 @@ def lib():
         "maragpu_destroy": (C.c_int, [_vp]),
         "maragpu_upload_prim": (C.c_int, [_vp, _dp, _sz]),
         "maragpu_download_prim": (C.c_int, [_vp, _dp, _sz]),
+        "maragpu_get_config": (C.c_int, [_vp, C.c_void_p]),
+        "maragpu_download_field": (C.c_int, [_vp, C.c_int, _dp, _sz]),
+        "maragpu_upload_field": (C.c_int, [_vp, C.c_int, _dp, _sz]),
         "maragpu_advance": (C.c_int, [_vp, C.c_double, C.POINTER(Stats)]),
         "maragpu_get_timestep": (C.c_double, [_vp, C.c_double]),
         "maragpu_measure_divB": (C.c_int, [_vp, _dp, _dp, _dp]),
@@ -325,6 +329,16 @@ class Context:
         P = np.empty(self.shape)
         check(lib().maragpu_download_prim(self.h, dptr(P), P.size), "download")
         return P
+
+    def download_field(self, q: int) -> np.ndarray:
+        """Field q's interior, shape N (the block of a prim/<field> dataset)."""
+        out = np.empty(tuple(self.cfg.N))
+        check(lib().maragpu_download_field(self.h, int(q), dptr(out), out.size), "download_field")
+        return out
+
+    def upload_field(self, q: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        check(lib().maragpu_upload_field(self.h, int(q), dptr(a), a.size), "upload_field")
 
     def advance(self, dt: float):
         st = Stats()

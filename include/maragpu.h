@@ -58,7 +58,8 @@ enum {
   MARAGPU_E_HIP = -2,       /* HIP runtime error */
   MARAGPU_E_NOMEM = -3,     /* device allocation failed */
   MARAGPU_E_SIZE = -4,      /* buffer size does not match the domain */
-  MARAGPU_E_NOMODEL = -5    /* learned c2p requested without loaded weights */
+  MARAGPU_E_NOMODEL = -5,   /* learned c2p requested without loaded weights */
+  MARAGPU_E_IO = -6         /* a checkpoint file could not be read or written */
 };
 
 typedef struct maragpu_config {
@@ -108,9 +109,18 @@ typedef struct maragpu_ctx maragpu_ctx;
 
 int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out);
 int maragpu_destroy(maragpu_ctx *ctx);
+/* the configuration the context was created with */
+int maragpu_get_config(const maragpu_ctx *ctx, maragpu_config *cfg);
 /* n = number of doubles = zones (ghosts included) x 8 */
 int maragpu_upload_prim(maragpu_ctx *ctx, const double *P_aos, size_t n);
 int maragpu_download_prim(maragpu_ctx *ctx, double *P_aos, size_t n);
+/* One primitive field q (0..7: rho, pre, vx, vy, vz, Bx, By, Bz) of the committed state,
+ * interior zones only, (i, j, k) order with k fastest; n = N0 N1 N2. This is the block that
+ * DataManagerHDF5 writes to and reads from prim/<field> (modules/unigrid.lua:85-212,243-292).
+ * The upload also writes the field's periodic ghost images. Uploading all eight fields
+ * gives a committed state, as maragpu_upload_prim does. */
+int maragpu_download_field(maragpu_ctx *ctx, int q, double *out, size_t n);
+int maragpu_upload_field(maragpu_ctx *ctx, int q, const double *in, size_t n);
 int maragpu_advance(maragpu_ctx *ctx, double dt, maragpu_stats *stats);
 /* CFL * min dx / MaxLambda of the last step: +inf before any solve, as in Mara */
 double maragpu_get_timestep(maragpu_ctx *ctx, double cfl);
