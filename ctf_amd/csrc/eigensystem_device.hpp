@@ -14,7 +14,8 @@
  *     null space of the known dimension), by complete-pivot elimination;
  *   R = M R_P with unit max-norm columns, L = R^-1; a state whose L exceeds 1e8 or whose
  *   |L R - I| exceeds 1e-8 is rejected (the caller then projects with the identity).
- * Matrices are 8x8 row-major in the lane's private memory; every loop bound is fixed.
+ * Eight lanes share one state (g8 below): each holds one column of every 8x8 matrix in
+ * registers.
  */
 #pragma once
 
@@ -118,76 +119,6 @@ __device__ __forceinline__ void state_fluxes(const D1 *P, D1 *U, D1 *F, double g
   }
 }
 
-/* A X = Bm for 8x8 A and 8 right-hand sides (both row-major, overwritten, X in Bm):
- * Gauss-Jordan elimination with partial pivoting. Returns 1 on a zero pivot. */
-__device__ inline int solve8(double *A, double *Bm)
-{
-#pragma clang fp contract(off)
-  for (int c = 0; c < 8; ++c) {
-    int piv = c;
-    double best = fabs(A[c * 8 + c]);
-    for (int r = c + 1; r < 8; ++r) {
-      const double a = fabs(A[r * 8 + c]);
-      if (a > best) { best = a; piv = r; }
-    }
-    if (!(best > 0.0)) return 1;
-    if (piv != c)
-      for (int k = 0; k < 8; ++k) { swapd(A[c * 8 + k], A[piv * 8 + k]); swapd(Bm[c * 8 + k], Bm[piv * 8 + k]); }
-    const double id = 1.0 / A[c * 8 + c];
-    for (int r = 0; r < 8; ++r) {
-      if (r == c) continue;
-      const double f = A[r * 8 + c] * id;
-      if (f == 0.0) continue;
-      for (int k = c; k < 8; ++k) A[r * 8 + k] -= f * A[c * 8 + k];
-      for (int k = 0; k < 8; ++k) Bm[r * 8 + k] -= f * Bm[c * 8 + k];
-    }
-  }
-  for (int r = 0; r < 8; ++r) {
-    const double id = 1.0 / A[r * 8 + r];
-    for (int k = 0; k < 8; ++k) Bm[r * 8 + k] *= id;
-  }
-  return 0;
-}
-
-/* `want` null vectors of the 8x8 matrix A (destroyed), whose null space has that
- * dimension: 8 - want complete-pivot elimination steps, then back substitution with each
- * free column set to one in turn, into columns col0.. of V. Returns 1 on a zero pivot. */
-__device__ inline int null_space8(double *A, double *V, int col0, int want)
-{
-#pragma clang fp contract(off)
-  int rowp[8], colp[8];
-  for (int i = 0; i < 8; ++i) { rowp[i] = i; colp[i] = i; }
-  const int rank = 8 - want;
-  for (int k = 0; k < rank; ++k) {
-    int pr = k, pc = k;
-    double best = -1.0;
-    for (int r = k; r < 8; ++r)
-      for (int c = k; c < 8; ++c) {
-        const double a = fabs(A[rowp[r] * 8 + colp[c]]);
-        if (a > best) { best = a; pr = r; pc = c; }
-      }
-    if (!(best > 0.0)) return 1;
-    { const int t = rowp[k]; rowp[k] = rowp[pr]; rowp[pr] = t; }
-    { const int t = colp[k]; colp[k] = colp[pc]; colp[pc] = t; }
-    const double piv = A[rowp[k] * 8 + colp[k]];
-    for (int r = k + 1; r < 8; ++r) {
-      const double f = A[rowp[r] * 8 + colp[k]] / piv;
-      for (int c = k; c < 8; ++c) A[rowp[r] * 8 + colp[c]] -= f * A[rowp[k] * 8 + colp[c]];
-    }
-  }
-  for (int fc = 0; fc < want; ++fc) {
-    double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    x[colp[rank + fc]] = 1.0;
-    for (int k = rank - 1; k >= 0; --k) {
-      double s = 0.0;
-      for (int c = k + 1; c < 8; ++c) s += A[rowp[k] * 8 + colp[c]] * x[colp[c]];
-      x[colp[k]] = -s / A[rowp[k] * 8 + colp[k]];
-    }
-    for (int i = 0; i < 8; ++i) V[i * 8 + col0 + fc] = x[i];
-  }
-  return 0;
-}
-
 /* Newton on the factored characteristic quartic from x; from outside the roots it moves
  * monotonically onto the extreme one, from a deflation estimate it polishes */
 __device__ inline double quartic_newton(double K, double L, double b0, double bi, double v, double x)
@@ -206,31 +137,222 @@ __device__ inline double quartic_newton(double K, double L, double b0, double bi
   return x;
 }
 
-/* L, R (64 doubles each, row-major) and the eigenvalues lam[8] (ascending) of the state P
- * for the flux along DIM. Returns 0 when usable, 1 on a singular Jacobian, 2 when a
- * cluster's null space could not be formed, 3 when the eigenvectors are too
- * ill-conditioned. */
+/* --------------------------------------------------------------------------
+ * The eigensystem, computed by 8 lanes together (an aligned group of 8 lanes of a
+ * wave, one state per group). Lane k of the group holds column k of every 8x8 matrix, so a
+ * lane keeps 8 doubles per matrix in registers instead of 64 in private memory:
+ *   column k of M = dU/dP and of B = dF/dP by one forward-mode pass along P[k];
+ *   Gauss-Jordan elimination with the pivot column broadcast to the group; each lane
+ *     updates its own column;
+ *   complete pivoting for the null spaces by a group reduction of (|a|, row, column);
+ *     back substitution by group sums.
+ * Every branch depends only on values that all lanes of the group hold alike, so the group
+ * stays converged. The eigenvalues are computed redundantly by each lane. Products and sums
+ * run in the order of a serial row-major implementation (the oracle's), except the
+ * back-substitution sums, which are group sums (round-off differences only).
+ * Outputs are row-wise: lane r returns row r of L and of R.
+ * ------------------------------------------------------------------------ */
+namespace g8 {
+
+__device__ __forceinline__ double bc(double v, int src) { return __shfl(v, src, 8); }
+__device__ __forceinline__ double sum(double v)
+{
+#pragma clang fp contract(off)
+  v = v + __shfl_xor(v, 1, 8);
+  v = v + __shfl_xor(v, 2, 8);
+  return v + __shfl_xor(v, 4, 8);
+}
+__device__ __forceinline__ double vmax(double v)
+{
+  v = fmax(v, __shfl_xor(v, 1, 8));
+  v = fmax(v, __shfl_xor(v, 2, 8));
+  return fmax(v, __shfl_xor(v, 4, 8));
+}
+__device__ __forceinline__ int imax(int v)
+{
+  v = max(v, __shfl_xor(v, 1, 8));
+  v = max(v, __shfl_xor(v, 2, 8));
+  return max(v, __shfl_xor(v, 4, 8));
+}
+__device__ __forceinline__ int imin(int v)
+{
+  v = min(v, __shfl_xor(v, 1, 8));
+  v = min(v, __shfl_xor(v, 2, 8));
+  return min(v, __shfl_xor(v, 4, 8));
+}
+
+/* a value the optimizer cannot trace back to the array element it came from: without it,
+ * the select chains below are folded back into an indexed load or store, which puts the
+ * array in private memory */
+__device__ __forceinline__ double opaque(double x)
+{
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+/* a[i] for a run-time i, by selects (no private-memory array) */
+__device__ __forceinline__ double pick(const double (&a)[8], int i)
+{
+  double v = opaque(a[0]);
+#pragma unroll
+  for (int k = 1; k < 8; ++k) v = (i == k) ? opaque(a[k]) : v;
+  return v;
+}
+__device__ __forceinline__ void put(double (&a)[8], int i, double v)
+{
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double cur = opaque(a[k]);
+    a[k] = (i == k) ? v : cur;
+  }
+}
+__device__ __forceinline__ void swap_rows(double (&a)[8], int i, int j)
+{
+  const double ai = pick(a, i), aj = pick(a, j);
+  put(a, i, aj);
+  put(a, j, ai);
+}
+
+/* lane r receives row r of the matrix whose column k lane k holds */
+__device__ __forceinline__ void transpose(const double (&col)[8], double (&row)[8], int lane)
+{
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const double send = pick(col, (lane - s) & 7);      // the element the lane (lane - s) & 7 wants
+    const double got = bc(send, (lane + s) & 7);
+    put(row, (lane + s) & 7, got);
+  }
+}
+
+/* A X = B, both held column-wise (lane k: column k), X into B: Gauss-Jordan elimination
+ * with partial pivoting. Returns 1 (in every lane) on a zero pivot. */
+__device__ __forceinline__ int solve(double (&A)[8], double (&B)[8], int lane)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    double col[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) col[r] = bc(A[r], c);
+    int piv = c;
+    double best = fabs(col[c]);
+#pragma unroll
+    for (int r = c + 1; r < 8; ++r) {
+      const double a = fabs(col[r]);
+      if (a > best) { best = a; piv = r; }
+    }
+    if (!(best > 0.0)) return 1;
+    swap_rows(A, c, piv);
+    swap_rows(B, c, piv);
+    swap_rows(col, c, piv);
+    const double id = 1.0 / col[c];
+    const double ac = A[c], bcv = B[c];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (r == c) continue;
+      const double f = col[r] * id;
+      if (f == 0.0) continue;
+      if (lane >= c) A[r] -= f * ac;
+      B[r] -= f * bcv;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const double id = 1.0 / bc(A[r], r);
+    B[r] *= id;
+  }
+  return 0;
+}
+
+/* `want` null vectors of the matrix held column-wise in A (destroyed), whose null space has
+ * that dimension: 8 - want complete-pivot elimination steps, then back substitution with each
+ * free column set to one in turn.
+ * Lane j writes component j of null vector fc into RProw[col0 + fc] (so lane j holds row j
+ * of the eigenvector matrix). Returns 1 (in every lane) on a zero pivot. */
+__device__ __forceinline__ int null_space(double (&A)[8], double (&RProw)[8], int col0, int want, int lane)
+{
+#pragma clang fp contract(off)
+  int lpos = lane;                       // this column's logical position
+  const int rank = 8 - want;
+  for (int k = 0; k < rank; ++k) {
+    // the first largest |a| over logical rows >= k (rows are swapped physically) and logical
+    // columns >= k, in row-major order: (|a| desc, row asc, logical column asc)
+    double best = -1.0;
+    int br = 8;
+    if (lpos >= k) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const double a = fabs(A[r]);
+        if (r >= k && a > best) { best = a; br = r; }
+      }
+    }
+    int bcl = lpos;
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1) {
+      const double ob = __shfl_xor(best, m, 8);
+      const int orr = __shfl_xor(br, m, 8), oc = __shfl_xor(bcl, m, 8);
+      const bool take = ob > best || (ob == best && (orr < br || (orr == br && oc < bcl)));
+      if (take) { best = ob; br = orr; bcl = oc; }
+    }
+    if (!(best > 0.0)) return 1;
+    swap_rows(A, k, br);
+    if (lpos == bcl) lpos = k;
+    else if (lpos == k) lpos = bcl;
+    const int src = imin(lpos == k ? lane : 8);
+    double col[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) col[r] = bc(A[r], src);
+    const double piv = pick(col, k);
+    const double ak = pick(A, k);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      if (r > k) {
+        const double f = col[r] / piv;
+        A[r] -= f * ak;
+      }
+    }
+  }
+  for (int fc = 0; fc < want; ++fc) {
+    double x = (lpos == rank + fc) ? 1.0 : 0.0;
+    for (int k = rank - 1; k >= 0; --k) {
+      const double akj = pick(A, k);
+      const double s = sum(lpos > k ? akj * x : 0.0);
+      if (lpos == k) x = -s / akj;
+    }
+    put(RProw, col0 + fc, x);
+  }
+  return 0;
+}
+
+/* L, R and the eigenvalues lam[8] (ascending) of the state P for the flux along DIM, for one
+ * state per group of 8 lanes; lane r returns Lrow = L[r][:], Rrow = R[r][:] and all of
+ * lam. Returns 0 when usable, 1 on a singular Jacobian, 2 when a cluster's null space could
+ * not be formed, 3 when the eigenvectors are too ill-conditioned; alike in every lane. */
 template <int DIM>
-__device__ int rmhd_eigensystem(const double *P, double *Lm, double *Rm, double *lam, double gamma)
+__device__ __forceinline__ int rmhd_eigensystem(const double *P, int lane, double (&Lrow)[8], double (&Rrow)[8], double (&lam)[8],
+                                double gamma)
 {
 #pragma clang fp contract(off)
   constexpr int VN = DIM == 1 ? vx : DIM == 2 ? vy : vz;
   constexpr int BN = DIM == 1 ? Bx : DIM == 2 ? By : Bz;
-  double M[64], X[64], W[64], RP[64];
-  // M = dU/dP into M, B = dF/dP into X, column k along P[k]
-  for (int k = 0; k < 8; ++k) {
+  double Mc[8], Xc[8];
+  {
     D1 Pd[8], Ud[8], Fd[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) Pd[i] = D1{P[i], i == k ? 1.0 : 0.0};
+    for (int i = 0; i < 8; ++i) Pd[i] = D1{P[i], i == lane ? 1.0 : 0.0};
     state_fluxes<DIM>(Pd, Ud, Fd, gamma);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      M[i * 8 + k] = Ud[i].d;
-      X[i * 8 + k] = Fd[i].d;
+      Mc[i] = Ud[i].d;
+      Xc[i] = Fd[i].d;
     }
   }
-  for (int i = 0; i < 64; ++i) W[i] = M[i];
-  if (solve8(W, X)) return 1;                    // X = A_P = M^-1 B
+  {
+    double Wc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Wc[i] = Mc[i];
+    if (solve(Wc, Xc, lane)) return 1;           // Xc: column of A_P = M^-1 B
+  }
 
   double ev[8];
   {
@@ -259,7 +381,6 @@ __device__ int rmhd_eigensystem(const double *P, double *Lm, double *Rm, double 
     const double lm = fmax((v * (1.0 - w2) - rt) * den - 1e-12, -1.0);
     const double r1 = quartic_newton(K, Lq, b0, bi, v, lp);
     const double r2 = quartic_newton(K, Lq, b0, bi, v, lm);
-    // the quartic's coefficients A4 x^4 + A3 x^3 + A2 x^2 + ..., deflated by (x-r1)(x-r2)
     const double A4 = K - Lq - b0 * b0;
     const double A3 = -4 * K * v + Lq * v * 2 + 2 * b0 * bi;
     const double A2 = 6 * K * v * v + Lq * (1.0 - v * v) + b0 * b0 - bi * bi;
@@ -274,57 +395,92 @@ __device__ int rmhd_eigensystem(const double *P, double *Lm, double *Rm, double 
     ev[6] = quartic_newton(K, Lq, b0, bi, v, (-c1 + sq) / (2.0 * A4));
     ev[7] = quartic_newton(K, Lq, b0, bi, v, (-c1 - sq) / (2.0 * A4));
   }
-  for (int i = 1; i < 8; ++i)                    // insertion sort, ascending
-    for (int j = i; j > 0 && ev[j] < ev[j - 1]; --j) swapd(ev[j], ev[j - 1]);
+  // ascending, by an odd-even transposition network (fixed indices: registers only)
+#pragma unroll
+  for (int pass = 0; pass < 8; ++pass)
+#pragma unroll
+    for (int i = pass & 1; i + 1 < 8; i += 2)
+      if (ev[i + 1] < ev[i]) swapd(ev[i], ev[i + 1]);
 
+  double RPr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < 8;) {
     int j = i + 1;
-    while (j < 8 && ev[j] - ev[j - 1] < 1e-7) ++j;
+    while (j < 8 && pick(ev, j) - pick(ev, j - 1) < 1e-7) ++j;
     double mean = 0.0;
-    for (int k = i; k < j; ++k) mean += ev[k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k >= i && k < j) mean += ev[k];
     mean /= (j - i);
-    for (int r = 0; r < 8; ++r)
-      for (int c = 0; c < 8; ++c) W[r * 8 + c] = X[r * 8 + c] - (r == c ? mean : 0.0);
-    if (null_space8(W, RP, i, j - i)) return 2;
-    for (int k = i; k < j; ++k) ev[k] = mean;
+    double Ac[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) Ac[r] = Xc[r] - (r == lane ? mean : 0.0);
+    if (null_space(Ac, RPr, i, j - i, lane)) return 2;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (k >= i && k < j) ev[k] = mean;
     i = j;
   }
-  // R = M R_P with unit max-norm columns
-  for (int r = 0; r < 8; ++r)
-    for (int c = 0; c < 8; ++c) {
-      double s = 0.0;
-      for (int k = 0; k < 8; ++k) s += M[r * 8 + k] * RP[k * 8 + c];
-      Rm[r * 8 + c] = s;
-    }
-  for (int c = 0; c < 8; ++c) {
-    double m = 0.0;
-    for (int r = 0; r < 8; ++r) m = fmax(m, fabs(Rm[r * 8 + c]));
-    if (!(m > 0.0)) return 1;
-    for (int r = 0; r < 8; ++r) Rm[r * 8 + c] /= m;
+  // R = M R_P, column-wise: lane c needs column c of R_P (a transpose of the rows) and
+  // every column of M (broadcast)
+  double RPc[8], Rc[8];
+  transpose(RPr, RPc, lane);
+#pragma unroll
+  for (int r = 0; r < 8; ++r) Rc[r] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) Rc[r] += bc(Mc[r], k) * RPc[k];
   }
-  for (int i = 0; i < 64; ++i) { W[i] = Rm[i]; Lm[i] = (i / 8 == i % 8) ? 1.0 : 0.0; }
-  if (solve8(W, Lm)) return 1;
+  {
+    double m = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) m = fmax(m, fabs(Rc[r]));
+    if (imax(!(m > 0.0) ? 1 : 0)) return 1;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) Rc[r] /= m;
+  }
+  double Lc[8];
+  {
+    double Wc[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) { Wc[r] = Rc[r]; Lc[r] = (r == lane) ? 1.0 : 0.0; }
+    if (solve(Wc, Lc, lane)) return 1;
+  }
   double lmax = 0.0, err = 0.0;
-  for (int r = 0; r < 8; ++r)
-    for (int c = 0; c < 8; ++c) {
-      lmax = fmax(lmax, fabs(Lm[r * 8 + c]));
-      double s = 0.0;
-      for (int k = 0; k < 8; ++k) s += Lm[r * 8 + k] * Rm[k * 8 + c];
-      err = fmax(err, fabs(s - (r == c ? 1.0 : 0.0)));
+  {
+    double LR[8] = {0, 0, 0, 0, 0, 0, 0, 0};     // column `lane` of L R
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) LR[r] += bc(Lc[r], k) * Rc[k];
     }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      lmax = fmax(lmax, fabs(Lc[r]));
+      err = fmax(err, fabs(LR[r] - (r == lane ? 1.0 : 0.0)));
+    }
+    lmax = vmax(lmax);
+    err = vmax(err);
+  }
   if (!(lmax < 1e8) || !(err < 1e-8)) return 3;
+  transpose(Lc, Lrow, lane);
+  transpose(Rc, Rrow, lane);
+#pragma unroll
   for (int k = 0; k < 8; ++k) lam[k] = ev[k];
   return 0;
 }
 
-/* the projection used where the eigensystem is rejected: the identity */
-__device__ inline void identity_eigensystem(double *Lm, double *Rm, double *lam)
+/* rows of the identity projection */
+__device__ __forceinline__ void identity_rows(int lane, double (&Lrow)[8], double (&Rrow)[8], double (&lam)[8])
 {
-  for (int i = 0; i < 8; ++i) {
-    lam[i] = 0.0;
-    for (int j = 0; j < 8; ++j) Lm[i * 8 + j] = Rm[i * 8 + j] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    lam[k] = 0.0;
+    Lrow[k] = Rrow[k] = (k == lane) ? 1.0 : 0.0;
   }
 }
+
+}  // namespace g8
 
 }  // namespace eig
 }  // namespace maragpu

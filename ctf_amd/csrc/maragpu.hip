@@ -864,7 +864,9 @@ __global__ __launch_bounds__(kBlock) void k_flux_weno_split(const double *__rest
  * it is rejected), the 6-zone split fluxes projected with L, the finite-difference WENO5
  * of every characteristic field, and the projection of their sum back with R
  * (Mara/weno-split.cpp:176-203, the eigensystem call at :181). In the mixed form (R =
- * float) only the zone fluxes and speeds are FP32. */
+ * float) only the zone fluxes and speeds are FP32.
+ * Eight lanes share one interface (eig::g8): lane q holds row q of L and R, projects and
+ * reconstructs characteristic field q, and writes flux component q. */
 template <typename R, int DIM>
 __global__ __launch_bounds__(kBlock) void k_flux_weno_split_char(const double *__restrict__ Uw,
                                                                  const double *__restrict__ P,
@@ -876,7 +878,8 @@ __global__ __launch_bounds__(kBlock) void k_flux_weno_split_char(const double *_
   const long c0 = DIM == 1 ? g.N[0] + 1 : g.n[0];
   const long c1 = DIM == 2 ? g.N[1] + 1 : g.n[1];
   const long c2 = DIM == 3 ? g.N[2] + 1 : g.n[2];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long t = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;     // alike in the 8 lanes
+  const int lane = (int)(threadIdx.x & 7);
   if (t >= c0 * c1 * c2) return;
   const long k = t % c2 + (DIM == 3 ? g.Ng - 1 : 0);
   const long j = (t / c2) % c1 + (DIM == 2 ? g.Ng - 1 : 0);
@@ -885,14 +888,14 @@ __global__ __launch_bounds__(kBlock) void k_flux_weno_split_char(const double *_
   const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
   const long z = (i * g.n[1] + j) * g.n[2] + k;
 
-  double Lm[64], Rm[64], lam[8];
+  double Lrow[8], Rrow[8], lam[8];
   {
     double Ph[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) Ph[q] = 0.5 * (P[q * T + z] + P[q * T + z + S]);
-    if (eig::rmhd_eigensystem<DIM>(Ph, Lm, Rm, lam, pp.gamma) != 0) {
-      eig::identity_eigensystem(Lm, Rm, lam);
-      atomicAdd(&st->eig_identity, 1ull);
+    if (eig::g8::rmhd_eigensystem<DIM>(Ph, lane, Lrow, Rrow, lam, pp.gamma) != 0) {
+      eig::g8::identity_rows(lane, Lrow, Rrow, lam);
+      if (lane == 0) atomicAdd(&st->eig_identity, 1ull);
     }
   }
   R mlr = AZ[z - 2 * S];
@@ -905,41 +908,26 @@ __global__ __launch_bounds__(kBlock) void k_flux_weno_split_char(const double *_
   // characteristic fields carry large parts that cancel on the way back through R, so
   // their rounding is multiplied by the condition number of R
   const double ml = (double)mlr;
-  double fp[8][6], fm[8][6];
-#pragma unroll 1
+  double fp[6], fm[6];
+#pragma unroll
   for (int m = 0; m < 6; ++m) {
     const long zm = z + (m - 2) * S;
-    double Fp[8], Fm[8];
+    double a = 0.0, b = 0.0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const double F = (double)FZ[q * T + zm], U = Uw[q * T + zm];
-      Fp[q] = 0.5 * (F + ml * U);
-      Fm[q] = 0.5 * (F - ml * U);
+    for (int c = 0; c < 8; ++c) {
+      const double F = (double)FZ[c * T + zm], U = Uw[c * T + zm];
+      a += Lrow[c] * (0.5 * (F + ml * U));
+      b += Lrow[c] * (0.5 * (F - ml * U));
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        a += Lm[q * 8 + c] * Fp[c];
-        b += Lm[q * 8 + c] * Fm[c];
-      }
-      fp[q][m] = a;
-      fm[q][m] = b;
-    }
+    fp[m] = a;
+    fm[m] = b;
   }
-  double f[8];
+  const double f = weno5<double, true, true>(fp[0], fp[1], fp[2], fp[3], fp[4], pp.IS, pp.sz10A) +
+                   weno5<double, false, true>(fm[1], fm[2], fm[3], fm[4], fm[5], pp.IS, pp.sz10A);
+  double a = 0.0;
 #pragma unroll
-  for (int q = 0; q < 8; ++q)
-    f[q] = weno5<double, true, true>(fp[q][0], fp[q][1], fp[q][2], fp[q][3], fp[q][4], pp.IS, pp.sz10A) +
-           weno5<double, false, true>(fm[q][1], fm[q][2], fm[q][3], fm[q][4], fm[q][5], pp.IS, pp.sz10A);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    double a = 0.0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) a += Rm[q * 8 + c] * f[c];
-    Fout[q * T + z] = (R)a;
-  }
+  for (int c = 0; c < 8; ++c) a += Rrow[c] * eig::g8::bc(f, c);
+  Fout[lane * T + z] = (R)a;
 }
 
 /* --------------------------------------------------------------------------
@@ -1743,15 +1731,24 @@ template <int DIM>
 __global__ __launch_bounds__(kBlock) void kb_eigensystem(long n, const double *P, double *L, double *R,
                                                          double *lam, int *ok, double gamma)
 {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 3;   // one state per 8 lanes
+  const int lane = (int)(threadIdx.x & 7);
   if (i >= n) return;
-  double p[8], Lm[64], Rm[64], ev[8];
+  double p[8], Lrow[8], Rrow[8], ev[8];
+#pragma unroll
   for (int q = 0; q < 8; ++q) p[q] = P[8 * i + q];
-  const int rc = eig::rmhd_eigensystem<DIM>(p, Lm, Rm, ev, gamma);
-  if (rc != 0) eig::identity_eigensystem(Lm, Rm, ev);
-  for (int q = 0; q < 64; ++q) { L[64 * i + q] = Lm[q]; R[64 * i + q] = Rm[q]; }
-  for (int q = 0; q < 8; ++q) lam[8 * i + q] = ev[q];
-  ok[i] = rc == 0;
+  const int rc = eig::g8::rmhd_eigensystem<DIM>(p, lane, Lrow, Rrow, ev, gamma);
+  if (rc != 0) eig::g8::identity_rows(lane, Lrow, Rrow, ev);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    L[64 * i + 8 * lane + q] = Lrow[q];
+    R[64 * i + 8 * lane + q] = Rrow[q];
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lam[8 * i + q] = ev[q];
+    ok[i] = rc == 0;
+  }
 }
 
 __global__ void kb_reconstruct(long n, const double *V, double *out, int op, double theta, int IS,
@@ -1919,7 +1916,7 @@ void launch_weno_split(maragpu_ctx *c, const PhysParams &pp)
   const long cnt = (long)(DIM == 1 ? g.N[0] + 1 : g.n[0]) * (DIM == 2 ? g.N[1] + 1 : g.n[1]) *
                    (DIM == 3 ? g.N[2] + 1 : g.n[2]);
   if (c->cfg.eigen == MARAGPU_EIG_RMHD)
-    k_flux_weno_split_char<R, DIM><<<nblocks(cnt), kBlock, 0, c->stream>>>(
+    k_flux_weno_split_char<R, DIM><<<nblocks(8 * cnt), kBlock, 0, c->stream>>>(
         c->Uw, c->Pw, FZ, AZ, static_cast<R *>(c->F[DIM - 1]), g, pp, c->dstats);
   else
     k_flux_weno_split<R, R, DIM><<<nblocks(cnt), kBlock, 0, c->stream>>>(c->Uw, FZ, AZ,
@@ -2650,9 +2647,9 @@ int maragpu_batch_eigensystem(size_t n, const double *P, double *L, double *R, d
   HIP_CHECK(hipMemcpy(dP.p, P, n * 64, hipMemcpyHostToDevice));
   auto *p = dP.get<double>(); auto *l = dL.get<double>(); auto *r = dR.get<double>();
   auto *v = dV.get<double>(); auto *k = dK.get<int>();
-  if (dim == 1) kb_eigensystem<1><<<nblocks(n), kBlock>>>((long)n, p, l, r, v, k, gamma);
-  if (dim == 2) kb_eigensystem<2><<<nblocks(n), kBlock>>>((long)n, p, l, r, v, k, gamma);
-  if (dim == 3) kb_eigensystem<3><<<nblocks(n), kBlock>>>((long)n, p, l, r, v, k, gamma);
+  if (dim == 1) kb_eigensystem<1><<<nblocks(8 * n), kBlock>>>((long)n, p, l, r, v, k, gamma);
+  if (dim == 2) kb_eigensystem<2><<<nblocks(8 * n), kBlock>>>((long)n, p, l, r, v, k, gamma);
+  if (dim == 3) kb_eigensystem<3><<<nblocks(8 * n), kBlock>>>((long)n, p, l, r, v, k, gamma);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpy(L, dL.p, n * 512, hipMemcpyDeviceToHost));
   HIP_CHECK(hipMemcpy(R, dR.p, n * 512, hipMemcpyDeviceToHost));
