@@ -154,31 +154,61 @@ __device__ inline double quartic_newton(double K, double L, double b0, double bi
  * ------------------------------------------------------------------------ */
 namespace g8 {
 
+/* any lane of the group to all of it (ds_bpermute) */
 __device__ __forceinline__ double bc(double v, int src) { return __shfl(v, src, 8); }
+
+/* Partner exchanges for group reductions, as DPP moves (a few cycles instead of an LDS
+ * round trip): lane i and i^1, lane i and i^2 (quad permutations), then lane i and 7-i
+ * (half-row mirror), which pairs the two quads of the group. Each is an involution, so both
+ * partners combine the same two values. */
+constexpr int kDppXor1 = 0xB1;       // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;       // quad_perm [2, 3, 0, 1]
+constexpr int kDppMirror = 0x141;    // row_half_mirror
+template <int CTRL> __device__ __forceinline__ int xchg(int v)
+{
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ double xchg(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)xchg<CTRL>((int)(unsigned)(b & 0xffffffffll));
+  const unsigned hi = (unsigned)xchg<CTRL>((int)(unsigned)((unsigned long long)b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 __device__ __forceinline__ double sum(double v)
 {
 #pragma clang fp contract(off)
-  v = v + __shfl_xor(v, 1, 8);
-  v = v + __shfl_xor(v, 2, 8);
-  return v + __shfl_xor(v, 4, 8);
+  v = v + xchg<kDppXor1>(v);
+  v = v + xchg<kDppXor2>(v);
+  return v + xchg<kDppMirror>(v);
 }
 __device__ __forceinline__ double vmax(double v)
 {
-  v = fmax(v, __shfl_xor(v, 1, 8));
-  v = fmax(v, __shfl_xor(v, 2, 8));
-  return fmax(v, __shfl_xor(v, 4, 8));
+  v = fmax(v, xchg<kDppXor1>(v));
+  v = fmax(v, xchg<kDppXor2>(v));
+  return fmax(v, xchg<kDppMirror>(v));
 }
 __device__ __forceinline__ int imax(int v)
 {
-  v = max(v, __shfl_xor(v, 1, 8));
-  v = max(v, __shfl_xor(v, 2, 8));
-  return max(v, __shfl_xor(v, 4, 8));
+  v = max(v, xchg<kDppXor1>(v));
+  v = max(v, xchg<kDppXor2>(v));
+  return max(v, xchg<kDppMirror>(v));
 }
 __device__ __forceinline__ int imin(int v)
 {
-  v = min(v, __shfl_xor(v, 1, 8));
-  v = min(v, __shfl_xor(v, 2, 8));
-  return min(v, __shfl_xor(v, 4, 8));
+  v = min(v, xchg<kDppXor1>(v));
+  v = min(v, xchg<kDppXor2>(v));
+  return min(v, xchg<kDppMirror>(v));
+}
+
+/* the first largest (value, row, column) of the group in the order: value desc, row asc,
+ * column asc; one partner exchange */
+template <int CTRL> __device__ __forceinline__ void pivot_step(double &best, int &br, int &bcl)
+{
+  const double ob = xchg<CTRL>(best);
+  const int orr = xchg<CTRL>(br), oc = xchg<CTRL>(bcl);
+  const bool take = ob > best || (ob == best && (orr < br || (orr == br && oc < bcl)));
+  if (take) { best = ob; br = orr; bcl = oc; }
 }
 
 /* a value the optimizer cannot trace back to the array element it came from: without it,
@@ -287,13 +317,9 @@ __device__ __forceinline__ int null_space(double (&A)[8], double (&RProw)[8], in
       }
     }
     int bcl = lpos;
-#pragma unroll
-    for (int m = 1; m < 8; m <<= 1) {
-      const double ob = __shfl_xor(best, m, 8);
-      const int orr = __shfl_xor(br, m, 8), oc = __shfl_xor(bcl, m, 8);
-      const bool take = ob > best || (ob == best && (orr < br || (orr == br && oc < bcl)));
-      if (take) { best = ob; br = orr; bcl = oc; }
-    }
+    pivot_step<kDppXor1>(best, br, bcl);
+    pivot_step<kDppXor2>(best, br, bcl);
+    pivot_step<kDppMirror>(best, br, bcl);
     if (!(best > 0.0)) return 1;
     swap_rows(A, k, br);
     if (lpos == bcl) lpos = k;
