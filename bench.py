@@ -371,6 +371,11 @@ def main():
     if tr:
         roofline["traffic"] = tr.get("bytes_per_launch")
         roofline["traffic_source"] = tr.get("source")
+        # the same launch against HBM: measured bytes (the profile's counters) over the
+        # live launch time; the kernel is not HBM-bound, this says how far it is from it
+        if roofline["traffic"]:
+            gbs = roofline["traffic"] / avg_flux_s / 1e9
+            roofline["hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
 
     forms_out = {}
     for name, r in results.items():
