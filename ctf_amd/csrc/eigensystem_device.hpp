@@ -144,12 +144,13 @@ __device__ inline double quartic_newton(double K, double L, double b0, double bi
  *   column k of M = dU/dP and of B = dF/dP by one forward-mode pass along P[k];
  *   Gauss-Jordan elimination with the pivot column broadcast to the group; each lane
  *     updates its own column;
- *   complete pivoting for the null spaces by a group reduction of (|a|, row, column);
- *     back substitution by group sums.
- * Every branch depends only on values that all lanes of the group hold alike, so the group
- * stays converged. The eigenvalues are computed redundantly by each lane. Products and sums
- * run in the order of a serial row-major implementation (the oracle's), except the
- * back-substitution sums, which are group sums (round-off differences only).
+ *   the eigenvectors lane-parallel: lane i eliminates its own full copy of A_P - lambda_i I
+ *     with complete pivoting (every lane of a cluster forms the same null space and keeps
+ *     its own vector of it).
+ * Every branch that encloses a group exchange depends only on values that all lanes of the
+ * group hold alike, so the group is converged at each exchange (the lane-local null vector
+ * has no exchange inside). The eigenvalues are computed redundantly by each lane. Products and sums
+ * run in the order of a serial row-major implementation.
  * Outputs are row-wise: lane r returns row r of L and of R.
  * ------------------------------------------------------------------------ */
 namespace g8 {
@@ -175,13 +176,6 @@ template <int CTRL> __device__ __forceinline__ double xchg(double v)
   const unsigned hi = (unsigned)xchg<CTRL>((int)(unsigned)((unsigned long long)b >> 32));
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-__device__ __forceinline__ double sum(double v)
-{
-#pragma clang fp contract(off)
-  v = v + xchg<kDppXor1>(v);
-  v = v + xchg<kDppXor2>(v);
-  return v + xchg<kDppMirror>(v);
-}
 __device__ __forceinline__ double vmax(double v)
 {
   v = fmax(v, xchg<kDppXor1>(v));
@@ -194,23 +188,6 @@ __device__ __forceinline__ int imax(int v)
   v = max(v, xchg<kDppXor2>(v));
   return max(v, xchg<kDppMirror>(v));
 }
-__device__ __forceinline__ int imin(int v)
-{
-  v = min(v, xchg<kDppXor1>(v));
-  v = min(v, xchg<kDppXor2>(v));
-  return min(v, xchg<kDppMirror>(v));
-}
-
-/* the first largest (value, row, column) of the group in the order: value desc, row asc,
- * column asc; one partner exchange */
-template <int CTRL> __device__ __forceinline__ void pivot_step(double &best, int &br, int &bcl)
-{
-  const double ob = xchg<CTRL>(best);
-  const int orr = xchg<CTRL>(br), oc = xchg<CTRL>(bcl);
-  const bool take = ob > best || (ob == best && (orr < br || (orr == br && oc < bcl)));
-  if (take) { best = ob; br = orr; bcl = oc; }
-}
-
 /* a value the optimizer cannot trace back to the array element it came from: without it,
  * the select chains below are folded back into an indexed load or store, which puts the
  * array in private memory */
@@ -294,59 +271,110 @@ __device__ __forceinline__ int solve(double (&A)[8], double (&B)[8], int lane)
   return 0;
 }
 
-/* `want` null vectors of the matrix held column-wise in A (destroyed), whose null space has
- * that dimension: 8 - want complete-pivot elimination steps, then back substitution with each
- * free column set to one in turn.
- * Lane j writes component j of null vector fc into RProw[col0 + fc] (so lane j holds row j
- * of the eigenvector matrix). Returns 1 (in every lane) on a zero pivot. */
-__device__ __forceinline__ int null_space(double (&A)[8], double (&RProw)[8], int col0, int want, int lane)
+/* Step K of the complete-pivot elimination in one lane (K a compile-time constant, so every
+ * index of the 8x8 array is one and the array stays in registers): the first largest |a|
+ * of the trailing block in row-major order, rows K <-> pr and columns K <-> pc swapped by
+ * selects, colp's nibbles K and pc exchanged, the rows below eliminated. Returns false on a
+ * zero pivot. */
+template <int K>
+__device__ __forceinline__ bool elim_step(double (&a)[8][8], unsigned &colp)
 {
 #pragma clang fp contract(off)
-  int lpos = lane;                       // this column's logical position
+  double best = -1.0;
+  int pr = K, pc = K;
+#pragma unroll
+  for (int r = K; r < 8; ++r)
+#pragma unroll
+    for (int c = K; c < 8; ++c) {
+      const double v = fabs(a[r][c]);
+      if (v > best) { best = v; pr = r; pc = c; }
+    }
+  if (!(best > 0.0)) return false;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {                 // rows K <-> pr
+    const double ak = opaque(a[K][c]);
+    double ap = ak;
+#pragma unroll
+    for (int r = K + 1; r < 8; ++r) ap = (pr == r) ? opaque(a[r][c]) : ap;
+#pragma unroll
+    for (int r = K + 1; r < 8; ++r) {
+      const double cur = opaque(a[r][c]);
+      a[r][c] = (pr == r) ? ak : cur;
+    }
+    a[K][c] = ap;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {                 // columns K <-> pc
+    const double ak = opaque(a[r][K]);
+    double ap = ak;
+#pragma unroll
+    for (int c = K + 1; c < 8; ++c) ap = (pc == c) ? opaque(a[r][c]) : ap;
+#pragma unroll
+    for (int c = K + 1; c < 8; ++c) {
+      const double cur = opaque(a[r][c]);
+      a[r][c] = (pc == c) ? ak : cur;
+    }
+    a[r][K] = ap;
+  }
+  const unsigned nk = (colp >> (4 * K)) & 15u, np = (colp >> (4 * pc)) & 15u;
+  colp = (colp & ~((15u << (4 * K)) | (15u << (4 * pc)))) | (np << (4 * K)) | (nk << (4 * pc));
+  const double piv = a[K][K];
+#pragma unroll
+  for (int r = K + 1; r < 8; ++r) {
+    const double f = a[r][K] / piv;
+#pragma unroll
+    for (int c = K; c < 8; ++c) a[r][c] -= f * a[K][c];
+  }
+  return true;
+}
+
+/* back substitution row K (compile-time), when K < rank */
+template <int K>
+__device__ __forceinline__ void back_step(const double (&a)[8][8], double (&xl)[8], int rank)
+{
+#pragma clang fp contract(off)
+  if (K < rank) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = K + 1; c < 8; ++c) s += a[K][c] * xl[c];
+    xl[K] = -s / a[K][K];
+  }
+}
+
+/* Null vector number fc of (a - mu I), whose null space has dimension want, in one lane:
+ * a is a full 8x8 copy of the matrix in the lane's registers (destroyed). 8 - want
+ * complete-pivot elimination steps, then back substitution with free column rank + fc set
+ * to one and the other free columns to zero. x receives the vector in the original column
+ * order. Returns 1 on a zero pivot. */
+__device__ __forceinline__ int null_vector(double (&a)[8][8], double mu, int want, int fc, double (&x)[8])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int r = 0; r < 8; ++r) a[r][r] = a[r][r] - mu;
   const int rank = 8 - want;
-  for (int k = 0; k < rank; ++k) {
-    // the first largest |a| over logical rows >= k (rows are swapped physically) and logical
-    // columns >= k, in row-major order: (|a| desc, row asc, logical column asc)
-    double best = -1.0;
-    int br = 8;
-    if (lpos >= k) {
+  unsigned colp = 0x76543210u;          // nibble c: the original column now at position c
+  bool ok = true;
+  if (ok && 0 < rank) ok = elim_step<0>(a, colp);
+  if (ok && 1 < rank) ok = elim_step<1>(a, colp);
+  if (ok && 2 < rank) ok = elim_step<2>(a, colp);
+  if (ok && 3 < rank) ok = elim_step<3>(a, colp);
+  if (ok && 4 < rank) ok = elim_step<4>(a, colp);
+  if (ok && 5 < rank) ok = elim_step<5>(a, colp);
+  if (ok && 6 < rank) ok = elim_step<6>(a, colp);
+  if (!ok) return 1;
+  double xl[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  put(xl, rank + fc, 1.0);
+  back_step<6>(a, xl, rank);
+  back_step<5>(a, xl, rank);
+  back_step<4>(a, xl, rank);
+  back_step<3>(a, xl, rank);
+  back_step<2>(a, xl, rank);
+  back_step<1>(a, xl, rank);
+  back_step<0>(a, xl, rank);
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const double a = fabs(A[r]);
-        if (r >= k && a > best) { best = a; br = r; }
-      }
-    }
-    int bcl = lpos;
-    pivot_step<kDppXor1>(best, br, bcl);
-    pivot_step<kDppXor2>(best, br, bcl);
-    pivot_step<kDppMirror>(best, br, bcl);
-    if (!(best > 0.0)) return 1;
-    swap_rows(A, k, br);
-    if (lpos == bcl) lpos = k;
-    else if (lpos == k) lpos = bcl;
-    const int src = imin(lpos == k ? lane : 8);
-    double col[8];
+  for (int c = 0; c < 8; ++c) x[c] = 0.0;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) col[r] = bc(A[r], src);
-    const double piv = pick(col, k);
-    const double ak = pick(A, k);
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      if (r > k) {
-        const double f = col[r] / piv;
-        A[r] -= f * ak;
-      }
-    }
-  }
-  for (int fc = 0; fc < want; ++fc) {
-    double x = (lpos == rank + fc) ? 1.0 : 0.0;
-    for (int k = rank - 1; k >= 0; --k) {
-      const double akj = pick(A, k);
-      const double s = sum(lpos > k ? akj * x : 0.0);
-      if (lpos == k) x = -s / akj;
-    }
-    put(RProw, col0 + fc, x);
-  }
+  for (int c = 0; c < 8; ++c) put(x, (int)((colp >> (4 * c)) & 15u), xl[c]);
   return 0;
 }
 
@@ -428,28 +456,40 @@ __device__ __forceinline__ int rmhd_eigensystem(const double *P, int lane, doubl
     for (int i = pass & 1; i + 1 < 8; i += 2)
       if (ev[i + 1] < ev[i]) swapd(ev[i], ev[i + 1]);
 
-  double RPr[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int i = 0; i < 8;) {
-    int j = i + 1;
-    while (j < 8 && pick(ev, j) - pick(ev, j - 1) < 1e-7) ++j;
-    double mean = 0.0;
+  // clusters: maximal runs of eigenvalues closer than 1e-7. Lane i forms eigenvector i, as
+  // vector (i - first) of its cluster's null space, from its own copy of A_P
+  double mean[8];
+  int first = 0, last = 8;
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k >= i && k < j) mean += ev[k];
-    mean /= (j - i);
-    double Ac[8];
+  for (int k = 0; k < 8; ++k) {
+    int s0 = 0, s1 = 8;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) Ac[r] = Xc[r] - (r == lane ? mean : 0.0);
-    if (null_space(Ac, RPr, i, j - i, lane)) return 2;
+    for (int j = 1; j < 8; ++j) {
+      const bool cut = !(ev[j] - ev[j - 1] < 1e-7);
+      if (cut && j <= k) s0 = j;
+      if (cut && j > k && s1 == 8) s1 = j;
+    }
+    double m = 0.0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (k >= i && k < j) ev[k] = mean;
-    i = j;
+    for (int q = 0; q < 8; ++q)
+      if (q >= s0 && q < s1) m += ev[q];
+    mean[k] = m / (s1 - s0);
+    if (k == lane) { first = s0; last = s1; }
   }
-  // R = M R_P, column-wise: lane c needs column c of R_P (a transpose of the rows) and
-  // every column of M (broadcast)
-  double RPc[8], Rc[8];
-  transpose(RPr, RPc, lane);
+  double RPc[8];                                 // column `lane` of R_P
+  {
+    double a[8][8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int r = 0; r < 8; ++r) a[r][c] = bc(Xc[r], c);
+    const int bad = null_vector(a, pick(mean, lane), last - first, lane - first, RPc);
+    if (imax(bad)) return 2;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ev[k] = mean[k];
+  // R = M R_P, column-wise: every column of M is broadcast
+  double Rc[8];
 #pragma unroll
   for (int r = 0; r < 8; ++r) Rc[r] = 0.0;
 #pragma unroll
