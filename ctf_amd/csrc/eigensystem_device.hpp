@@ -1,4 +1,4 @@
-/* Characteristic eigensystem of the RMHD flux Jacobian, FP64, one state per lane.
+/* Characteristic eigensystem of the RMHD flux Jacobian, FP64, one state per group of 8 lanes.
  *
  * Used by the weno-split sweep to project the split fluxes into characteristic fields
  * (WenoSplit::intercell_flux_sweep, Mara/weno-split.cpp:163-207, which calls
