@@ -3,11 +3,12 @@ oracle's, which the CPU suite pins by invariants (L R = I, A R = R diag(lam) for
 independently built conserved Jacobian, the known eigenvalues).
 
 The eigenvalues come from closed forms and Newton iterations that both sides evaluate in
-the same order, so they agree to round-off. The eigenvectors come from centred-difference
-Jacobians (step 1e-6), whose round-off (~1e-10 relative) is amplified by 1/gap between
-neighbouring eigenvalues; eigenvectors of a cluster are a basis of its eigenspace, fixed
-only up to that choice. So the comparison is basis-free: the conserved Jacobian
-R diag(lam) L, and column by column (up to sign) only where the spectrum is well separated.
+the same order, so they agree to round-off. The eigenvectors come from forward-mode
+Jacobians and eliminations whose round-off is amplified by 1/gap between neighbouring
+eigenvalues (the device runs one state on 8 cooperating lanes, with its own pivot ties and
+sum orders); eigenvectors of a cluster are a basis of its eigenspace, fixed only up to that
+choice. So the comparison is basis-free: the conserved Jacobian R diag(lam) L, and column
+by column (up to sign) only where the spectrum is well separated.
 """
 import os
 import sys
