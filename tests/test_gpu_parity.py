@@ -377,12 +377,13 @@ def test_mara_verbs_weno_split_characteristic(gpu, oracle_lib):
 
 
 @pytest.mark.parametrize("riemann,advance,v_rms", [("hlld", "single", 0.4), ("hllc", "rk3", 0.2),
-                                                  ("hll", "rk3", 0.4), ("hllc", "rk2", 0.4)])
+                                                  ("hll", "rk3", 0.4), ("hllc", "rk2", 0.4),
+                                                  ("hlld", "rk4", 0.4)])
 def test_plm_muscl_steps_vs_oracle(gpu, oracle_lib, riemann, advance, v_rms):
     """plm-muscl (the CTU-Hancock operator) on seeded turbulence against the oracle, which is
     pinned bitwise to the reference's PlmCtuHancockOperator: the predictor's half-step face
     states, the transverse corrections and the corrector fluxes, under each Runge-Kutta
-    scheme (the operator receives the full step dt in every stage).
+    scheme, RK4 included (the operator receives the full step dt in every stage).
 
     HLLC with CTU is fragile where the normal field changes sign across a face (its star
     state divides by the small HLL normal field): at v_rms = 0.4 the reference itself fails
