@@ -258,7 +258,7 @@ def weno_split_record(w, N, steps):
     return rec
 
 
-def plm_muscl_record(w, N, steps):
+def plm_muscl_record(w, N, steps, cpu=True):
     """The plm-muscl operator (CTU-Hancock: predictor face fluxes, a Hancock half step per
     axis, corrector fluxes) against plm-split with the same PLM reconstruction, HLLD and
     Runge-Kutta scheme, on the same turbulence at N^3, FP64: `steps` timed steps after the
@@ -279,6 +279,27 @@ def plm_muscl_record(w, N, steps):
                      "update_ms_per_step": r["update_ms_per_step"], "max_divB": r["max_divB"],
                      "riemann_solves": st.get("hlld_success", 0) + st.get("hlld_failure", 0)}
     rec["plm_muscl"]["L1_vs_plm_split"] = l1_vs(ctu["P"], split["P"], w.Ng)
+    # the reference's own PlmCtuHancockOperator on the host, one thread, beside it
+    sys.path.insert(0, ROOT)
+    from oracle.oracle import Lib, Setup, have_ref
+    if cpu and have_ref():
+        n = 32
+        lib = Lib("ref")
+        so = lib.solver(Setup(N=(n,) * 3, godunov="plm-muscl", riemann=base.riemann, advance=base.advance,
+                              gamma=w.gamma))
+        P = native.init_turbulence(**w.resized(n).ic_kwargs())
+        so.advance(P, 0.0)
+        steps_cpu = 0
+        t = time.perf_counter()
+        while steps_cpu < 2:
+            e, _ = so.advance(P, so.get_timestep(w.cfl))
+            steps_cpu += 1
+            if e:
+                break
+        el = time.perf_counter() - t
+        rec["cpu_reference_zone_updates_per_s"] = n ** 3 * steps_cpu / el
+        rec["cpu_sample"] = (f"reference PlmCtuHancockOperator ({base.riemann}/{base.advance}) at {n}^3, "
+                             f"{steps_cpu} step(s) after the priming step, {el:.1f} s, one thread")
     return rec
 
 
@@ -406,7 +427,7 @@ def main():
         ws = weno_split_record(w, args.weno_split_grid, args.weno_split_steps)
     pm = None
     if args.plm_muscl_grid:
-        pm = plm_muscl_record(w, args.plm_muscl_grid, args.plm_muscl_steps)
+        pm = plm_muscl_record(w, args.plm_muscl_grid, args.plm_muscl_steps, cpu=not args.no_cpu_baseline)
     dr = None
     if not args.no_driving:
         log("[bench] driving and cooling")
