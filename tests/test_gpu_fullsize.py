@@ -8,7 +8,8 @@ size:
   - every wave speed is at most the speed of light;
   - a second run from the same state gives the same bits (all reductions are fixed-order
     or order-independent).
-Plain FP64 and the mixed form (whose FP32 fluxes still enter a conservative update)."""
+Plain FP64 and the mixed form (whose FP32 fluxes still enter a conservative update), and
+the other operators on the same turbulence."""
 import numpy as np
 import pytest
 
@@ -18,13 +19,13 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def c2_state(gpu):
+def c2_states(gpu):
     w = workloads.C2
-    return w, gpu.init_turbulence(**w.ic_kwargs())
+    return {N: gpu.init_turbulence(**w.resized(N).ic_kwargs()) for N in (128, 256)}
 
 
-def run(gpu, w, P0, precision, steps):
-    ctx = gpu.Context(gpu.make_config(**workloads.replace(w, precision=precision, c2p="plain").config_kwargs()))
+def run(gpu, w, P0, steps):
+    ctx = gpu.Context(gpu.make_config(**w.config_kwargs()))
     ctx.upload(P0)
     before = ctx.measure()["mean_cons"]
     div0 = ctx.divB()[2]
@@ -40,11 +41,20 @@ def run(gpu, w, P0, precision, steps):
     return out
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_c2_full_size_invariants(gpu, c2_state, precision):
-    w, P0 = c2_state
-    a = run(gpu, w, P0, precision, 3)
-    # conservation: D, tau, S, B means unchanged to round-off of 16.8 M-term sums
+# the headline (plm-split, FP64 and mixed) at its 256^3; plm-muscl at 256^3 as the bench runs
+# it; weno-split at the 128^3 of its bench record
+CASES = [("plm-split", "component", "fp64", 256), ("plm-split", "component", "mixed", 256),
+         ("plm-muscl", "component", "fp64", 256), ("weno-split", "component", "fp64", 128),
+         ("weno-split", "rmhd", "fp64", 128)]
+
+
+@pytest.mark.parametrize("godunov,eigen,precision,N", CASES)
+def test_c2_full_size_invariants(gpu, c2_states, godunov, eigen, precision, N):
+    w = workloads.replace(workloads.C2.resized(N), godunov=godunov, eigen=eigen, precision=precision,
+                          c2p="plain")
+    P0 = c2_states[N]
+    a = run(gpu, w, P0, 3)
+    # conservation: D, tau, S, B means unchanged to round-off of the interior sums
     scale = np.maximum(np.abs(a["before"]), 1.0)
     assert (np.abs(a["after"] - a["before"]) <= 1e-12 * scale).all(), a["after"] - a["before"]
     # div B stays at its initial round-off
@@ -53,5 +63,5 @@ def test_c2_full_size_invariants(gpu, c2_state, precision):
     assert all(0.0 < lam <= 1.0 for lam in a["lams"])
     assert np.isfinite(a["P"]).all()
     # determinism
-    b = run(gpu, w, P0, precision, 3)
+    b = run(gpu, w, P0, 3)
     assert np.array_equal(a["P"], b["P"]) and a["lams"] == b["lams"]
