@@ -1138,26 +1138,20 @@ __global__ __launch_bounds__(kBlock, 2) void k_ctu_predict(const double *__restr
   block_add<1>(cnt, dst);
 }
 
-/* the predictors' ghost zones from their periodic images (PeriodicBoundary::ApplyBoundaries) */
-__global__ __launch_bounds__(kBlock) void k_ctu_sync(double *__restrict__ Ux, double *__restrict__ Uy,
-                                                     double *__restrict__ Uz, GridDesc g)
+/* the interior zone whose periodic image z is (z itself inside): each coordinate wrapped
+ * once, as PeriodicBoundary::ApplyBoundaries fills the ghosts (needs N >= Ng per axis) */
+__device__ __forceinline__ long image_zone(const GridDesc &g, long z)
 {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  if (t >= ni) return;
-  const long z = interior_zone(g, t);
-  const int k = (int)(z % g.n[2]), j = (int)((z / g.n[2]) % g.n[1]), i = (int)(z / ((long)g.n[1] * g.n[2]));
-  double *A[3] = {Ux, Uy, Uz};
-  for (int d = 0; d < 3; ++d) {
-    double v[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) v[q] = A[d][q * g.T + z];
-    scatter_images(g, A[d], g.T, i, j, k, v);
-  }
+  int k = (int)(z % g.n[2]), j = (int)((z / g.n[2]) % g.n[1]), i = (int)(z / ((long)g.n[1] * g.n[2]));
+  i = i < g.Ng ? i + g.N[0] : (i >= g.Ng + g.N[0] ? i - g.N[0] : i);
+  j = j < g.Ng ? j + g.N[1] : (j >= g.Ng + g.N[1] ? j - g.N[1] : j);
+  k = k < g.Ng ? k + g.N[2] : (k >= g.Ng + g.N[2] ? k - g.N[2] : k);
+  return ((long)i * g.n[1] + j) * g.n[2] + k;
 }
 
-/* the predictors inverted (Rmhd::ConsToPrim) from P at every zone, ghosts included; a
- * failure keeps P there and counts once per axis */
+/* the predictors, synchronised (their ghost zones are their periodic images) and inverted
+ * (Rmhd::ConsToPrim) from P at every zone, ghosts included; a failure keeps P there and
+ * counts once per axis. A ghost zone reads its image's predictor instead of a copy of it. */
 __global__ __launch_bounds__(kBlock, 3) void k_ctu_c2p(const double *__restrict__ P, const double *__restrict__ Ux,
                                                        const double *__restrict__ Uy, const double *__restrict__ Uz,
                                                        double *__restrict__ Px, double *__restrict__ Py,
@@ -1170,11 +1164,12 @@ __global__ __launch_bounds__(kBlock, 3) void k_ctu_c2p(const double *__restrict_
   if (z < T) {
     const double *Ud[3] = {Ux, Uy, Uz};
     double *Pd[3] = {Px, Py, Pz};
+    const long zi = image_zone(g, z);
 #pragma unroll 1
     for (int d = 0; d < 3; ++d) {
       double u[8], p[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) { u[q] = Ud[d][q * T + z]; p[q] = P[q * T + z]; }
+      for (int q = 0; q < 8; ++q) { u[q] = Ud[d][q * T + zi]; p[q] = P[q * T + z]; }
       int iters = 0, which = 0;
       if (cons_to_prim(u, p, gamma, pfloor, true, iters, which) != 0) cnt[0] += 1;
 #pragma unroll
@@ -2246,7 +2241,6 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     faces(2, c->Pw, F1, 0, 0, fl + 0);
     faces(3, c->Pw, F2, 0, 0, fl + 0);
     k_ctu_predict<<<nblocks(T - sxz), kBlock, 0, c->stream>>>(c->Pw, c->Uw, F0, F1, F2, Ux, Uy, Uz, g, pp, dt, fl + 1);
-    k_ctu_sync<<<nblocks(ni), kBlock, 0, c->stream>>>(Ux, Uy, Uz, g);
     k_ctu_c2p<<<nblocks(T), kBlock, 0, c->stream>>>(c->Pw, Ux, Uy, Uz, Px, Py, Pz, g, c->cfg.gamma, c->cfg.pfloor,
                                                     c->dstats, fl + 2);
     faces(1, Px, F0, sxz, 1, fl + 3);
