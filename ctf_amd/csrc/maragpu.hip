@@ -1948,6 +1948,9 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
     if (cfg->N[d] < 1 || !(cfg->x1[d] > cfg->x0[d])) return MARAGPU_E_INVALID;
   const int need_ng = cfg->recon == MARAGPU_WENO5 ? 3 : cfg->recon == MARAGPU_PLM ? 2 : 1;
   if (cfg->Ng < need_ng || cfg->Ng > 8) return MARAGPU_E_INVALID;
+  // every ghost zone is the periodic image of an interior zone one period away
+  for (int d = 0; d < 3; ++d)
+    if (cfg->N[d] < cfg->Ng) return MARAGPU_E_INVALID;
   if (cfg->rk < 1 || cfg->rk > 4 || cfg->riemann < 0 || cfg->riemann > 2 || cfg->recon < 0 ||
       cfg->recon > 2 || cfg->precision < 0 || cfg->precision > 1 || !(cfg->gamma > 1.0))
     return MARAGPU_E_INVALID;

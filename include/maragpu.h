@@ -107,6 +107,8 @@ typedef struct maragpu_stats {
 
 typedef struct maragpu_ctx maragpu_ctx;
 
+/* MARAGPU_E_INVALID for a malformed configuration, including an axis with fewer interior
+ * zones than ghosts (every ghost zone is the periodic image of an interior one) */
 int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out);
 int maragpu_destroy(maragpu_ctx *ctx);
 /* the configuration the context was created with */

@@ -136,3 +136,15 @@ def test_lua_verbs_reject_like_mara():
     with pytest.raises(MaraError):
         Mara.set_domain((0, 0), (1, 1), (4, 4), 8, 3)   # 2-D is outside this build
     Mara.close()
+
+
+def test_create_rejects_axes_shorter_than_the_ghosts():
+    """Configuration checks run before any device call, so they hold without a GPU: a grid
+    axis with fewer interior zones than ghosts has ghosts that are not one-period images."""
+    import ctypes as C
+    from ctf_amd import native
+    for N in ((2, 8, 8), (8, 2, 8), (8, 8, 1)):
+        cfg = native.make_config(N=N, Ng=3)
+        h = C.c_void_p()
+        assert native.lib().maragpu_create(C.byref(cfg), C.byref(h)) == -1
+        assert not h.value
