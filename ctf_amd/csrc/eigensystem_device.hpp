@@ -433,8 +433,10 @@ __device__ __forceinline__ int rmhd_eigensystem(const double *P, int lane, doubl
     const double rt = sqrt(w2 * (1.0 - V2) * (1.0 - V2 * w2 - v * v * (1.0 - w2)));
     const double lp = fmin((v * (1.0 - w2) + rt) * den + 1e-12, 1.0);
     const double lm = fmax((v * (1.0 - w2) - rt) * den - 1e-12, -1.0);
-    const double r1 = quartic_newton(K, Lq, b0, bi, v, lp);
-    const double r2 = quartic_newton(K, Lq, b0, bi, v, lm);
+    // the two fast roots at once: even lanes from lp, odd lanes from lm (one Newton loop for
+    // the group instead of two; each root is the same as alone)
+    const double rr = quartic_newton(K, Lq, b0, bi, v, (lane & 1) ? lm : lp);
+    const double r1 = bc(rr, 0), r2 = bc(rr, 1);
     const double A4 = K - Lq - b0 * b0;
     const double A3 = -4 * K * v + Lq * v * 2 + 2 * b0 * bi;
     const double A2 = 6 * K * v * v + Lq * (1.0 - v * v) + b0 * b0 - bi * bi;
@@ -446,8 +448,9 @@ __device__ __forceinline__ int rmhd_eigensystem(const double *P, int lane, doubl
     const double sq = sqrt(disc);
     ev[4] = r1;
     ev[5] = r2;
-    ev[6] = quartic_newton(K, Lq, b0, bi, v, (-c1 + sq) / (2.0 * A4));
-    ev[7] = quartic_newton(K, Lq, b0, bi, v, (-c1 - sq) / (2.0 * A4));
+    const double rs = quartic_newton(K, Lq, b0, bi, v, (lane & 1) ? (-c1 - sq) / (2.0 * A4) : (-c1 + sq) / (2.0 * A4));
+    ev[6] = bc(rs, 0);
+    ev[7] = bc(rs, 1);
   }
   // ascending, by an odd-even transposition network (fixed indices: registers only)
 #pragma unroll
