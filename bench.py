@@ -11,7 +11,7 @@ reports zone-updates/s, per-field L1 difference from the plain FP64 state and ma
 
 value = interior zones x timed steps / wall seconds of the plain FP64 form.
 
-Printed: one JSON line (rank 0). Multi-GPU runs are not part of this build.
+Printed: one JSON line (rank 0). It measures one GPU; --gpus above 1 exits with a message.
 """
 from __future__ import annotations
 
@@ -346,8 +346,8 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "launches": 3}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part "
-                                   "of it (see DESIGN.md for how the update shards)",
+        print(json.dumps({"error": "bench.py measures one GPU only: the update runs on a single "
+                                   "MI355X, and --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -26,7 +26,8 @@ HEADERS = ["rmhd_device.hpp", "mlp_device.hpp", "eigensystem_device.hpp", "ou_fi
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
     "-Wno-unused-value", "-Wno-unused-result",
-    # packed FP32 instructions off (see DESIGN: hardware trap on this pool)
+    # packed FP32 instructions off: some of their forms have given wrong results, without an
+    # error, on these GPUs
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # matrix-core accumulators in ordinary VGPRs: the network's activations are read by
     # VALU right after each product, which the accumulator-register form makes a copy

@@ -121,7 +121,7 @@ int ensure_ctx(bool for_advance = true)
 {
   App *a = app();
   if (!a->have_domain) return fail("[Mara] need a domain to run this, use set_domain");
-  if (!a->fluid_rmhd) return fail("[Mara] this build runs the 'rmhd' fluid only; use set_fluid('rmhd')");
+  if (!a->fluid_rmhd) return fail("[Mara] this library runs the 'rmhd' fluid only; use set_fluid('rmhd')");
   if (for_advance) {
     if (!a->have_godunov) return fail("[Mara] need a godunov operator, use set_godunov");
     // WenoSplit splits the fluxes itself (Mara/weno-split.cpp:163-207); only plm-split calls
@@ -210,7 +210,7 @@ int mara_set_fluid(const char *key)
     return 0;
   }
   if (std::strcmp(key, "euler") == 0 || std::strcmp(key, "srhd") == 0)
-    return fail(std::string("[Mara] fluid '") + key + "' is outside this build (rmhd only)");
+    return fail(std::string("[Mara] fluid '") + key + "' is not supported by this library (rmhd only)");
   return fail(std::string("[Mara] no such fluid '") + key + "'");
 }
 
@@ -225,7 +225,7 @@ int mara_set_eos(const char *key, double gamma)
   }
   if (key && (std::strcmp(key, "gamma-law + polytrope") == 0 || std::strcmp(key, "shen") == 0 ||
               std::strcmp(key, "tabulated") == 0))
-    return fail(std::string("[Mara] eos '") + key + "' is outside this build (gamma-law only)");
+    return fail(std::string("[Mara] eos '") + key + "' is not supported by this library (gamma-law only)");
   return fail(std::string("[Mara] no such eos '") + (key ? key : "(null)") + "'");
 }
 
@@ -239,7 +239,7 @@ int mara_set_boundary(const char *key)
   if (key && (std::strcmp(key, "outflow") == 0 || std::strcmp(key, "perxouty") == 0 ||
               std::strcmp(key, "reflect2d") == 0 || std::strcmp(key, "magnetic-bubble") == 0 ||
               std::strcmp(key, "wind-inflow") == 0))
-    return fail(std::string("[Mara] boundary '") + key + "' is outside this build (periodic only)");
+    return fail(std::string("[Mara] boundary '") + key + "' is not supported by this library (periodic only)");
   return fail(std::string("[Mara] no such boundary conditions '") + (key ? key : "(null)") + "'");
 }
 
@@ -394,7 +394,7 @@ int mara_set_domain(const double *x0, const double *x1, const int *N, int Nd, in
 {
   if (int rc = need_app()) return rc;
   if (!x0 || !x1 || !N) return fail("[Mara] set_domain: null argument");
-  if (Nd != 3) return fail("[Mara] this build runs 3D domains only");
+  if (Nd != 3) return fail("[Mara] this library runs 3D domains only");
   if (Nq != 8) return fail("[Mara] rmhd needs Nq = 8");
   maragpu_config &c = app()->cfg;
   for (int d = 0; d < 3; ++d) {
@@ -589,7 +589,7 @@ int mara_measure_max_lorentz_factor(double *max)
 int mara_new_ou_field(int Nd, double P0, double zeta, int k1, int seed, void *buf, size_t cap, size_t *len)
 {
   if (int rc = need_app()) return rc;
-  if (Nd != 3) return fail("[Mara] new_ou_field: this build drives 3-d domains only");
+  if (Nd != 3) return fail("[Mara] new_ou_field: this library drives 3-d domains only");
   maragpu_ou_field *f = nullptr;
   int rc = maragpu_ou_field_create(P0, zeta, k1, seed, &f);
   if (rc) return fail("[Mara] new_ou_field: invalid parameters", rc);

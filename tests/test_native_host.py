@@ -104,7 +104,7 @@ def test_lua_verbs_reject_like_mara():
     Mara.start()
     with pytest.raises(MaraError, match="no such fluid"):
         Mara.set_fluid("plasma")
-    with pytest.raises(MaraError, match="outside this build"):
+    with pytest.raises(MaraError, match="not supported by this library"):
         Mara.set_fluid("euler")
     with pytest.raises(MaraError, match="no such riemann"):
         Mara.set_riemann("roe")
@@ -134,7 +134,7 @@ def test_lua_verbs_reject_like_mara():
     Mara.set_fluid("rmhd")
     Mara.set_domain((0, 0, 0), (1, 1, 1), (4, 4, 4), 8, 3)
     with pytest.raises(MaraError):
-        Mara.set_domain((0, 0), (1, 1), (4, 4), 8, 3)   # 2-D is outside this build
+        Mara.set_domain((0, 0), (1, 1), (4, 4), 8, 3)   # 2-D is not supported
     Mara.close()
 
 
