@@ -1199,17 +1199,15 @@ enum StageKind {
 template <typename FT>
 __device__ __forceinline__ double ld(const FT *__restrict__ a, long i) { return (double)a[i]; }
 
+/* One interior zone z of the stage update: CT of the three flux arrays, the flux divergence
+ * and the RK combination; U (and A) written, the new U returned in unew */
 template <typename FT>
-__global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
-                                                   const FT *__restrict__ Fz,
-                                                   const double *__restrict__ U0, double *__restrict__ U,
-                                                   double *__restrict__ A, GridDesc g, double dt, int kind)
+__device__ __forceinline__ void update_zone(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
+                                            const FT *__restrict__ Fz, const double *__restrict__ U0,
+                                            double *__restrict__ U, double *__restrict__ A, const GridDesc &g,
+                                            long z, double dt, int kind, double *unew)
 {
 #pragma clang fp contract(off)
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  if (t >= ni) return;
-  const long z = interior_zone(g, t);
   const long T = g.T, sx = g.sx, sy = g.sy, sz = g.sz;
   const FT *FxBy = Fx + (long)By * T, *FxBz = Fx + (long)Bz * T;
   const FT *FyBx = Fy + (long)Bx * T, *FyBz = Fy + (long)Bz * T;
@@ -1258,17 +1256,77 @@ __global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, co
     const double L = -(F0[q]-F1[q])/g.dx - (G0[q]-G1[q])/g.dy - (H0[q]-H1[q])/g.dz;
     const long m = q * T + z;
     const double u0 = U0[m];
+    double un;
     switch (kind) {
-    case ST_EULER: U[m] = u0 + dt*L; break;
-    case ST_RK2_FINAL: U[m] = 0.5*(u0 + U[m] + dt*L); break;
-    case ST_RK3_2: U[m] = 3./4*u0 + 1./4*U[m] + 1./4*dt*L; break;
-    case ST_RK3_3: U[m] = 1./3*u0 + 2./3*U[m] + 2./3*dt*L; break;
-    case ST_RK4_1: { const double l = dt*L; A[m] = l; U[m] = u0 + 0.5*l; } break;
-    case ST_RK4_2: { const double l = dt*L; A[m] = A[m] + 2.0*l; U[m] = u0 + 0.5*l; } break;
-    case ST_RK4_3: { const double l = dt*L; A[m] = A[m] + 2.0*l; U[m] = u0 + 1.0*l; } break;
-    default: { const double l = dt*L; U[m] = u0 + (1.0/6.0)*(A[m] + l); } break;
+    case ST_EULER: un = u0 + dt*L; break;
+    case ST_RK2_FINAL: un = 0.5*(u0 + U[m] + dt*L); break;
+    case ST_RK3_2: un = 3./4*u0 + 1./4*U[m] + 1./4*dt*L; break;
+    case ST_RK3_3: un = 1./3*u0 + 2./3*U[m] + 2./3*dt*L; break;
+    case ST_RK4_1: { const double l = dt*L; A[m] = l; un = u0 + 0.5*l; } break;
+    case ST_RK4_2: { const double l = dt*L; A[m] = A[m] + 2.0*l; un = u0 + 0.5*l; } break;
+    case ST_RK4_3: { const double l = dt*L; A[m] = A[m] + 2.0*l; un = u0 + 1.0*l; } break;
+    default: { const double l = dt*L; un = u0 + (1.0/6.0)*(A[m] + l); } break;
     }
+    U[m] = un;
+    unew[q] = un;
   }
+}
+
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
+                                                   const FT *__restrict__ Fz,
+                                                   const double *__restrict__ U0, double *__restrict__ U,
+                                                   double *__restrict__ A, GridDesc g, double dt, int kind)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  if (t >= ni) return;
+  double unew[8];
+  update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, interior_zone(g, t), dt, kind, unew);
+}
+
+/* The stage update followed at once by the next conserved-to-primitive pass of the zone
+ * (plm-split, plain c2p, not the first pass): k_update and k_cons_to_prim<false> in one
+ * kernel, the new U taken from registers instead of read back. Same arithmetic, same
+ * counters. */
+template <typename FT>
+__global__ __launch_bounds__(kBlock, 3) void k_update_c2p(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
+                                                          const FT *__restrict__ Fz,
+                                                          const double *__restrict__ U0, double *__restrict__ U,
+                                                          double *__restrict__ A, double *__restrict__ P, GridDesc g,
+                                                          double dt, int kind, double gamma, double pfloor, int ereset,
+                                                          double preset, double dreset, DevStats *st, int pass)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  if (t < ni) {
+    const long T = g.T;
+    const int k = (int)(t % g.N[2]) + g.Ng;
+    const int j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+    const int i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+    const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
+    double u[8], p[8];
+    update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, z, dt, kind, u);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) p[q] = P[q * T + z];
+    int iters = 0, which = 0;
+    const int err = cons_to_prim(u, p, gamma, pfloor, true, iters, which);
+    int failed = err != 0;
+    if (failed && ereset) {
+      p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
+      p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
+      failed = 0;
+    }
+    const int mult = scatter_images(g, P, T, i, j, k, p);
+    cnt[0] = 1;
+    cnt[1] = (unsigned long long)iters;
+    cnt[2] = which != 0;
+    cnt[3] = failed ? (unsigned long long)mult : 0ull;
+  }
+  unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                       &st->stage_failed[pass] };
+  block_add<4>(cnt, dst);
 }
 
 /* luaC_mean_max_divB stencil, Mara/mara.cpp:2298-2314, over its loop range */
@@ -2251,16 +2309,36 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     faces(3, Pz, F2, sxz, 1, fl + 3);
     hipEventRecord(c->ev[p.b], c->stream);
   };
+  // plm-split with the plain c2p runs each update together with the c2p pass after it
+  const bool fuse = c->cfg.godunov == MARAGPU_PLM_SPLIT && c->cfg.c2p == MARAGPU_C2P_PLAIN;
+  auto update_c2p = [&](int kind, int pass) {
+    EventPair p = mark(eu);
+    hipEventRecord(c->ev[p.a], c->stream);
+    if (mixed)
+      k_update_c2p<float><<<nblocks(ni), kBlock, 0, c->stream>>>(
+          (const float *)c->F[0], (const float *)c->F[1], (const float *)c->F[2], c->U0, c->U, c->A, c->Pw, g, dt,
+          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass);
+    else
+      k_update_c2p<double><<<nblocks(ni), kBlock, 0, c->stream>>>(
+          (const double *)c->F[0], (const double *)c->F[1], (const double *)c->F[2], c->U0, c->U, c->A, c->Pw, g, dt,
+          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass);
+    hipEventRecord(c->ev[p.b], c->stream);
+  };
   int pass = 0;
   for (size_t s = 0; s < kinds.size(); ++s) {
-    int rc = c2p(s == 0, pass++);
-    if (rc) return rc;
+    if (s == 0 || !fuse) {
+      int rc = c2p(s == 0, pass++);
+      if (rc) return rc;
+    }
     if (ctu) ctu_fluxes((int)s);
     else sweeps();
-    update(kinds[s]);
+    if (fuse) update_c2p(kinds[s], pass++);
+    else update(kinds[s]);
   }
-  int rc = c2p(0, pass++);
-  if (rc) return rc;
+  if (!fuse) {
+    int rc = c2p(0, pass++);
+    if (rc) return rc;
+  }
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));
   const int ev_end = evi++;
