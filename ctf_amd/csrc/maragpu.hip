@@ -1286,16 +1286,17 @@ __global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, co
 }
 
 /* The stage update followed at once by the next conserved-to-primitive pass of the zone
- * (plm-split, plain c2p, not the first pass): k_update and k_cons_to_prim<false> in one
- * kernel, the new U taken from registers instead of read back. Same arithmetic, same
- * counters. */
+ * (plain c2p, not the first pass): k_update and k_cons_to_prim<false> in one kernel, the new
+ * U taken from registers instead of read back. Same arithmetic, same counters; with Uw
+ * (weno-split, plm-muscl) U's ghost images are written as k_cons_to_prim writes them. */
 template <typename FT>
 __global__ __launch_bounds__(kBlock, 3) void k_update_c2p(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
                                                           const FT *__restrict__ Fz,
                                                           const double *__restrict__ U0, double *__restrict__ U,
                                                           double *__restrict__ A, double *__restrict__ P, GridDesc g,
                                                           double dt, int kind, double gamma, double pfloor, int ereset,
-                                                          double preset, double dreset, DevStats *st, int pass)
+                                                          double preset, double dreset, DevStats *st, int pass,
+                                                          double *__restrict__ Uw)
 {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long ni = (long)g.N[0] * g.N[1] * g.N[2];
@@ -1319,6 +1320,10 @@ __global__ __launch_bounds__(kBlock, 3) void k_update_c2p(const FT *__restrict__
       failed = 0;
     }
     const int mult = scatter_images(g, P, T, i, j, k, p);
+    if (Uw) {
+      scatter_images(g, Uw, T, i, j, k, u);
+      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
+    }
     cnt[0] = 1;
     cnt[1] = (unsigned long long)iters;
     cnt[2] = which != 0;
@@ -2309,19 +2314,22 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     faces(3, Pz, F2, sxz, 1, fl + 3);
     hipEventRecord(c->ev[p.b], c->stream);
   };
-  // plm-split with the plain c2p runs each update together with the c2p pass after it
-  const bool fuse = c->cfg.godunov == MARAGPU_PLM_SPLIT && c->cfg.c2p == MARAGPU_C2P_PLAIN;
+  // with the plain c2p each update runs together with the c2p pass after it
+  const bool fuse = c->cfg.c2p == MARAGPU_C2P_PLAIN;
   auto update_c2p = [&](int kind, int pass) {
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
     if (mixed)
       k_update_c2p<float><<<nblocks(ni), kBlock, 0, c->stream>>>(
           (const float *)c->F[0], (const float *)c->F[1], (const float *)c->F[2], c->U0, c->U, c->A, c->Pw, g, dt,
-          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass);
+          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw);
     else
       k_update_c2p<double><<<nblocks(ni), kBlock, 0, c->stream>>>(
           (const double *)c->F[0], (const double *)c->F[1], (const double *)c->F[2], c->U0, c->U, c->A, c->Pw, g, dt,
-          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass);
+          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw);
+    if (c->Uw && c->cfg.ereset)
+      k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
+                                                                      &c->dstats->stage_reset[pass]);
     hipEventRecord(c->ev[p.b], c->stream);
   };
   int pass = 0;
