@@ -42,6 +42,10 @@ using namespace maragpu;
 namespace {
 
 constexpr int kBlock = 256;
+// block size of k_flux_hlld_fast: FP64 in blocks of 2 waves, so fewer waves wait at the
+// block's closing reduction (44.2-44.4 against 44.8 ms per plain 256^3 step; 1-wave blocks
+// ran at 75.9); FP32 in blocks of 4 (2-wave blocks ran its sweep at 3.8 ms against 2.0)
+template <typename R> constexpr int kFluxBlock = sizeof(R) == 4 ? 256 : 128;
 // occupancy targets of k_flux_hlld_fast (waves per SIMD), measured at 256^3 WENO5/HLLD:
 // FP64 at 2 (209 VGPRs, 40 B/lane scratch; 4.0 ms, against 4.7 at 3 with 184 B spilled),
 // FP32 at 5 (96 VGPRs, 44 B/lane; 2.00 ms, against 2.21 at 4 and 2.18 at 6)
@@ -115,10 +119,10 @@ __device__ __forceinline__ double wave_max(double v)
 }
 
 /* Sums NV counters over the block and adds them to dst[0..NV). */
-template <int NV>
+template <int NV, int BLOCK = kBlock>
 __device__ __forceinline__ void block_add(const unsigned long long (&v)[NV], unsigned long long *const (&dst)[NV])
 {
-  __shared__ unsigned long long red[kBlock / 64][NV];
+  __shared__ unsigned long long red[BLOCK / 64][NV];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
@@ -129,7 +133,7 @@ __device__ __forceinline__ void block_add(const unsigned long long (&v)[NV], uns
   if (threadIdx.x < NV) {
     unsigned long long s = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) s += red[w][threadIdx.x];
+    for (int w = 0; w < BLOCK / 64; ++w) s += red[w][threadIdx.x];
     if (s) atomicAdd(dst[threadIdx.x], s);
   }
 }
@@ -152,9 +156,10 @@ __device__ __forceinline__ bool queue_push(const RedoQueue &rq, bool push, long 
   return true;
 }
 
+template <int BLOCK = kBlock>
 __device__ __forceinline__ void block_max_lambda(double lam, unsigned long long *dst)
 {
-  __shared__ double red[kBlock / 64];
+  __shared__ double red[BLOCK / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // NaN never enters: fmax drops it, as `if (MaxLambda < ml)` does in the reference
   const double m = wave_max(lam);
@@ -163,7 +168,7 @@ __device__ __forceinline__ void block_max_lambda(double lam, unsigned long long 
   if (threadIdx.x == 0) {
     double b = 0.0;
 #pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) b = fmax(b, red[w]);
+    for (int w = 0; w < BLOCK / 64; ++w) b = fmax(b, red[w]);
     if (b > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(b));
   }
 }
@@ -714,7 +719,7 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 2) void k_flux(const d
  * kernel lowers its register peak. The queue holds every interface of the sweep, so it
  * cannot overflow. */
 template <typename R, int DIM>
-__global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? kFastFluxWaves32 : kFastFluxWaves64)
+__global__ __launch_bounds__(kFluxBlock<R>, sizeof(R) == 4 ? kFastFluxWaves32 : kFastFluxWaves64)
 void k_flux_hlld_fast(const double *__restrict__ P, R *__restrict__ Fout, GridDesc g, int recon, PhysParams pp,
                       DevStats *st, RedoQueue rq)
 {
@@ -739,9 +744,9 @@ void k_flux_hlld_fast(const double *__restrict__ P, R *__restrict__ Fout, GridDe
     lam_out = (double)lam;
   }
   queue_push(rq, defer, z);
-  block_max_lambda(lam_out, &st->lam_bits);
+  block_max_lambda<kFluxBlock<R>>(lam_out, &st->lam_bits);
   unsigned long long *const dst[3] = { &st->hlld_success, &st->hlld_c2p_iters, &st->hlld_newton_iters };
-  block_add<3>(cnt, dst);
+  block_add<3, kFluxBlock<R>>(cnt, dst);
 }
 
 /* The queued interfaces of a sweep, recomputed by the full FP64 method (reconstruction,
@@ -1955,7 +1960,7 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
     // the sweep, then its queued interfaces by the full FP64 solver: in the mixed form the
     // FP32 HLLD failures, in the plain form whatever left the common path
     hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
-    k_flux_hlld_fast<R, DIM><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats,
+    k_flux_hlld_fast<R, DIM><<<(unsigned)((n + kFluxBlock<R> - 1) / kFluxBlock<R>), kFluxBlock<R>, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats,
                                                                     c->redo);
     k_flux_redo<FT, DIM, RIEMANN_HLLD><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp,
                                                                               c->dstats, c->redo);
