@@ -1339,6 +1339,56 @@ __global__ __launch_bounds__(kBlock, 3) void k_update_c2p(const FT *__restrict__
   block_add<4>(cnt, dst);
 }
 
+/* The start of a step with the plain c2p: GodunovOperator::PrimToCons of the committed state
+ * (U0 and U, as k_prim_to_cons) and the first conserved-to-primitive pass of the zone (the
+ * zero-filled previous state, as k_cons_to_prim<false> with first = 1) in one kernel. */
+__global__ __launch_bounds__(kBlock, 5) void k_p2c_c2p(const double *__restrict__ Pc, double *__restrict__ U0,
+                                                      double *__restrict__ U, double *__restrict__ P, GridDesc g,
+                                                      double gamma, double pfloor, int ereset, double preset,
+                                                      double dreset, DevStats *st, int pass, double *__restrict__ Uw)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  unsigned long long cnt[4] = {0, 0, 0, 0};
+  if (t < ni) {
+    const long T = g.T;
+    const int k = (int)(t % g.N[2]) + g.Ng;
+    const int j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+    const int i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+    const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
+    double u[8], p[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) p[q] = Pc[q * T + z];
+    prim_to_cons(p, u, gamma - 1.0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      U0[q * T + z] = u[q];
+      U[q * T + z] = u[q];
+      p[q] = 0.0;
+    }
+    int iters = 0, which = 0;
+    const int err = cons_to_prim(u, p, gamma, pfloor, false, iters, which);
+    int failed = err != 0;
+    if (failed && ereset) {
+      p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
+      p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
+      failed = 0;
+    }
+    const int mult = scatter_images(g, P, T, i, j, k, p);
+    if (Uw) {
+      scatter_images(g, Uw, T, i, j, k, u);
+      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
+    }
+    cnt[0] = 1;
+    cnt[1] = (unsigned long long)iters;
+    cnt[2] = which != 0;
+    cnt[3] = failed ? (unsigned long long)mult : 0ull;
+  }
+  unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                       &st->stage_failed[pass] };
+  block_add<4>(cnt, dst);
+}
+
 /* luaC_mean_max_divB stencil, Mara/mara.cpp:2298-2314, over its loop range */
 /* The Mara.measure table (Mara/mara.cpp:1954-2262) over the interior zones of the committed
  * state: per zone the quantities of luaC_mean_velocity, _mean_cons, _mean_prim,
@@ -2195,7 +2245,10 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   HIP_CHECK(hipMemsetAsync(c->dstats, 0, sizeof(DevStats), c->stream));
   HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));
   const int ev_start = evi++;
-  k_prim_to_cons<<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, g, c->cfg.gamma);
+  // with the plain c2p each update runs together with the c2p pass after it, and the step's
+  // PrimToCons with its first pass
+  const bool fuse = c->cfg.c2p == MARAGPU_C2P_PLAIN;
+  if (!fuse) k_prim_to_cons<<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, g, c->cfg.gamma);
 
   auto c2p = [&](int first, int pass) -> int {
     EventPair p = mark(ec);
@@ -2319,8 +2372,6 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     faces(3, Pz, F2, sxz, 1, fl + 3);
     hipEventRecord(c->ev[p.b], c->stream);
   };
-  // with the plain c2p each update runs together with the c2p pass after it
-  const bool fuse = c->cfg.c2p == MARAGPU_C2P_PLAIN;
   auto update_c2p = [&](int kind, int pass) {
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
@@ -2337,9 +2388,21 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
                                                                       &c->dstats->stage_reset[pass]);
     hipEventRecord(c->ev[p.b], c->stream);
   };
+  auto p2c_c2p = [&](int pass) {
+    EventPair p = mark(ec);
+    hipEventRecord(c->ev[p.a], c->stream);
+    k_p2c_c2p<<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+                                                      c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass,
+                                                      c->Uw);
+    if (c->Uw && c->cfg.ereset)
+      k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
+                                                                      &c->dstats->stage_reset[pass]);
+    hipEventRecord(c->ev[p.b], c->stream);
+  };
   int pass = 0;
   for (size_t s = 0; s < kinds.size(); ++s) {
-    if (s == 0 || !fuse) {
+    if (s == 0 && fuse) p2c_c2p(pass++);
+    else if (s == 0 || !fuse) {
       int rc = c2p(s == 0, pass++);
       if (rc) return rc;
     }
