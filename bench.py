@@ -31,8 +31,12 @@ from ctf_amd import native, workloads  # noqa: E402
 
 METRIC = "zone-updates/s at 256^3 RMHD (plain vs learned-c2p); L1 err vs FP64"
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and matrix) peak
-FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (counts packed FP32, two per lane)
+FP32_UNPACKED_TFLOPS = 78.6  # the same ALUs without packed FP32, which this build does not use
+BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 matrix-core peak
 HBM_PEAK_GBS = 8000.0
+MLP_FLOPS_PER_ZONE = 20480   # matrix-core flops the network issues per zone (padded tiles)
+UPDATE_FLOPS_PER_ZONE = 150  # CT EMF averages, flux divergence and the RK combination
 FIELDS = ["rho", "pre", "vx", "vy", "vz", "Bx", "By", "Bz"]
 
 
@@ -53,6 +57,80 @@ def interfaces_per_sweep(w):
     T = n**3
     strides = (n * n, n, 1)
     return [T - 5 * s for s in strides]
+
+
+def sweep_flops(w, fc=None):
+    """Algorithmic flops of one sweep launch (mean over the three axes): the oracle's count
+    per interface along the device's path (nested c2p from the average state) times the
+    interfaces of the sweep."""
+    fc = fc or flop_counts(w)
+    fpi = fc.get("device_path", fc)["flops_per_interface"]
+    nif = interfaces_per_sweep(w)
+    return (nif[0] * fpi["x"] + nif[1] * fpi["y"] + nif[2] * fpi["z"]) / 3.0
+
+
+def stage_count(w):
+    return {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}[w.advance]
+
+
+def update_bytes_per_zone(w, flux_bytes):
+    """Algorithmic HBM bytes of one fused update + c2p pass per interior zone, averaged
+    over the stages: the three axes' 8 fluxes, U^n and the previous primitives read, U read
+    by the stages that combine it (all but the first), U and the primitives written."""
+    stages = stage_count(w)
+    reads_u = (stages - 1) / stages if w.advance in ("rk2", "rk3") else (1.0 if w.advance == "rk4" else 0.0)
+    return 3 * 8 * flux_bytes + 64 + 64 + 64 * reads_u + 64 + 64
+
+
+def form_roofline(w, r, form):
+    """The roofline view of one form's kernels from the HIP-event durations the library
+    records around them (averages over the timed steps)."""
+    fc = flop_counts(w)
+    nint = w.N**3
+    T = (w.N + 2 * w.Ng) ** 3
+    nif = sum(interfaces_per_sweep(w)) / 3.0
+    fp32 = form != "plain"
+    f_sweep = sweep_flops(w, fc)
+    t_sweep = r["flux_ms_per_launch"] * 1e-3
+    peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+    ach = f_sweep / t_sweep / 1e12
+    sweep = {"kernel": f"k_flux_hlld_fast<{'float' if fp32 else 'double'}> + k_flux_redo (one axis sweep)",
+             "bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+             "flops_per_launch": f_sweep, "avg_launch_ms": r["flux_ms_per_launch"],
+             "flops_source": "oracle count along the device's path (tests/golden/flops_*.json device_path)",
+             "bytes_per_launch": nif * (64 + (32 if fp32 else 64)),
+             "hbm_GBs": nif * (64 + (32 if fp32 else 64)) / t_sweep / 1e9}
+    if fp32:
+        sweep["peak_without_packed_fp32"] = FP32_UNPACKED_TFLOPS
+        sweep["frac_without_packed_fp32"] = ach / FP32_UNPACKED_TFLOPS
+    ub = update_bytes_per_zone(w, 4 if fp32 else 8) * nint
+    t_upd = r["update_ms_per_launch"] * 1e-3
+    upd = {"kernel": f"k_update_c2p<{'float' if fp32 else 'double'}, {'true' if form == 'learned' else 'false'}>",
+           "bound": "hbm", "achieved": ub / t_upd / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": ub / t_upd / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": ub, "avg_launch_ms": r["update_ms_per_launch"]}
+    pb = 256.0 * nint
+    t_p2c = r["c2p_ms_per_launch"] * 1e-3
+    p2c = {"kernel": f"k_p2c_c2p<{'true' if form == 'learned' else 'false'}>", "bound": "hbm",
+           "achieved": pb / t_p2c / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": pb / t_p2c / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": pb, "avg_launch_ms": r["c2p_ms_per_launch"]}
+    stages = stage_count(w)
+    c2p_zone = fc["flops_per_c2p_zone"]
+    step_flops = (3 * stages * f_sweep + (stages + 1) * nint * c2p_zone + stages * nint * UPDATE_FLOPS_PER_ZONE)
+    step_s = r["ms_per_step"] * 1e-3
+    step = {"flops_per_step": step_flops, "achieved": step_flops / step_s / 1e12, "peak": peak,
+            "unit": "TFLOP/s", "frac": step_flops / step_s / 1e12 / peak,
+            "what": "the step's algorithmic flops (sweeps, c2p passes at the reference cascade's count, "
+                    "updates) over its wall time"}
+    out = {"sweep": sweep, "update_c2p": upd, "p2c_c2p": p2c, "step": step}
+    if form == "learned":
+        mf = MLP_FLOPS_PER_ZONE * nint * (stages + 1)
+        t_l = (r["update_ms_per_launch"] * stages + r["c2p_ms_per_launch"]) * 1e-3
+        out["network"] = {"bound": "mfma", "achieved": mf / t_l / 1e12, "peak": BF16_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": mf / t_l / 1e12 / BF16_PEAK_TFLOPS,
+                          "flops_per_step": mf,
+                          "what": "the network's matrix-core flops over the time of the kernels it runs in "
+                                  "(which also do the update and the Newton polish)"}
+    return out
 
 
 def run_form(w, P0, steps, warmup, dts=None, learned_model=None, total_steps=0):
@@ -311,6 +389,58 @@ def traffic_record(w):
     return None
 
 
+def forms_record(w, forms, steps, warmup, acc_steps, model):
+    """Runs the plain FP64 form, then the mixed and mixed + learned forms on its dt
+    sequence; returns the per-form records and the plain run."""
+    log(f"[bench] initial data {w.name} {w.N}^3")
+    P0 = native.init_turbulence(**w.ic_kwargs())
+    results = {}
+    log(f"[bench] {w.name} plain FP64")
+    plain = run_form(w, P0, steps, warmup, total_steps=acc_steps)
+    results["plain"] = plain
+    dts = plain["dts"]
+    if "mixed" in forms:
+        log(f"[bench] {w.name} mixed precision")
+        wm = workloads.replace(w, precision="mixed")
+        results["mixed"] = run_form(wm, P0, steps, warmup, dts=dts, total_steps=acc_steps)
+    if "learned" in forms:
+        if not os.path.exists(model):
+            raise SystemExit(f"learned form requested but no network file at {model}")
+        log(f"[bench] {w.name} mixed + learned c2p")
+        wl = workloads.replace(w, precision="mixed", c2p="learned")
+        results["learned"] = run_form(wl, P0, steps, warmup, dts=dts, learned_model=model, total_steps=acc_steps)
+    del P0
+    forms_out = {}
+    for name, r in results.items():
+        rec = {"zone_updates_per_s": r["zone_updates_per_s"], "ms_per_step": r["ms_per_step"],
+               "speedup_vs_plain": r["zone_updates_per_s"] / plain["zone_updates_per_s"],
+               "flux_ms_per_step": r["flux_ms_per_step"], "c2p_ms_per_step": r["c2p_ms_per_step"],
+               "update_ms_per_step": r["update_ms_per_step"], "max_divB": r["max_divB"],
+               "priming_step_ms": r["prime_ms"]}
+        rec["accuracy_after_steps"] = r["steps_run"]
+        if name != "plain":
+            rec["L1_vs_fp64"] = l1_vs(r["P"], plain["P"], w.Ng)
+        st = r["stats"]
+        zones = max(st.get("c2p_zones", 0), 1)
+        rec["newton_iterations_per_zone"] = st.get("c2p_iterations", 0) / zones
+        rec["c2p_fallback_fraction"] = st.get("c2p_fallbacks", 0) / zones
+        rec["hlld_to_hllc_fallbacks"] = st.get("hlld_failure", 0)
+        rec["hlld_solves"] = st.get("hlld_success", 0) + st.get("hlld_failure", 0)
+        if rec["hlld_solves"]:
+            rec["hlld_newton_iterations_per_solve"] = st.get("hlld_newton_iterations", 0) / rec["hlld_solves"]
+        if st.get("fp64_redo", 0) or st.get("fp64_redo_overflow", 0):
+            rec["fp64_redo_interfaces"] = st.get("fp64_redo", 0)
+            rec["fp64_redo_overflow"] = st.get("fp64_redo_overflow", 0)
+        if name == "learned":
+            g = max(st.get("mlp_guesses", 0), 1)
+            rec["mlp_fallback_fraction"] = st.get("mlp_fallbacks", 0) / g
+        rec["roofline"] = form_roofline(w, r, name)
+        forms_out[name] = rec
+    for r in results.values():
+        r.pop("P", None)
+    return forms_out, plain
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,7 +452,10 @@ def main():
     ap.add_argument("--grid", type=int, default=None, help="override the workload's grid size")
     ap.add_argument("--forms", default="plain,mixed,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--weno-split-grid", type=int, default=128,
+    ap.add_argument("--no-c5", dest="c5", action="store_false",
+                    help="skip the c5 record (512^3, every form)")
+    ap.add_argument("--c5-steps", type=int, default=3)
+    ap.add_argument("--weno-split-grid", type=int, default=256,
                     help="grid of the weno-split measurement (0: skip it)")
     ap.add_argument("--weno-split-steps", type=int, default=3)
     ap.add_argument("--no-driving", action="store_true", help="skip the driving/cooling record")
@@ -355,37 +488,20 @@ def main():
     if args.grid:
         w = w.resized(args.grid)
     w = workloads.replace(w, precision="fp64", c2p="plain")
-    log(f"[bench] initial data {w.name} {w.N}^3")
-    P0 = native.init_turbulence(**w.ic_kwargs())
     forms = [f for f in args.forms.split(",") if f]
-    results = {}
-    log("[bench] plain FP64")
     acc_steps = w.steps if args.accuracy_steps is None else args.accuracy_steps
-    plain = run_form(w, P0, args.steps, args.warmup, total_steps=acc_steps)
-    results["plain"] = plain
-    dts = plain["dts"]
-    if "mixed" in forms:
-        log("[bench] mixed precision")
-        wm = workloads.replace(w, precision="mixed")
-        results["mixed"] = run_form(wm, P0, args.steps, args.warmup, dts=dts, total_steps=acc_steps)
-    if "learned" in forms:
-        if not os.path.exists(args.model):
-            raise SystemExit(f"learned form requested but no network file at {args.model}")
-        log("[bench] mixed + learned c2p")
-        wl = workloads.replace(w, precision="mixed", c2p="learned")
-        results["learned"] = run_form(wl, P0, args.steps, args.warmup, dts=dts,
-                                      learned_model=args.model, total_steps=acc_steps)
+    forms_out, plain = forms_record(w, forms, args.steps, args.warmup, acc_steps, args.model)
 
     fc = flop_counts(w)
-    nif = interfaces_per_sweep(w)
-    fpi = fc["flops_per_interface"]
-    flops_per_launch = (nif[0] * fpi["x"] + nif[1] * fpi["y"] + nif[2] * fpi["z"]) / 3.0
+    flops_per_launch = sweep_flops(w, fc)
     avg_flux_s = plain["flux_ms_per_launch"] * 1e-3
     achieved = flops_per_launch / avg_flux_s / 1e12
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                 "kernel": "k_flux_hlld_fast<double> + k_flux_redo (one axis sweep, WENO5/HLLD)",
                 "flops_per_launch": flops_per_launch,
+                "flops_source": "oracle count along the device's path (nested c2p from the average state), "
+                                "tests/golden/flops_%s.json" % w.name,
                 "avg_launch_ms": plain["flux_ms_per_launch"],
                 "traffic": None}
     tr = traffic_record(w)
@@ -398,29 +514,14 @@ def main():
             gbs = roofline["traffic"] / avg_flux_s / 1e9
             roofline["hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
 
-    forms_out = {}
-    for name, r in results.items():
-        rec = {"zone_updates_per_s": r["zone_updates_per_s"], "ms_per_step": r["ms_per_step"],
-               "speedup_vs_plain": r["zone_updates_per_s"] / plain["zone_updates_per_s"],
-               "flux_ms_per_step": r["flux_ms_per_step"], "c2p_ms_per_step": r["c2p_ms_per_step"],
-               "update_ms_per_step": r["update_ms_per_step"], "max_divB": r["max_divB"],
-               "priming_step_ms": r["prime_ms"]}
-        rec["accuracy_after_steps"] = r["steps_run"]
-        if name != "plain":
-            rec["L1_vs_fp64"] = l1_vs(r["P"], plain["P"], w.Ng)
-        st = r["stats"]
-        zones = max(st.get("c2p_zones", 0), 1)
-        rec["newton_iterations_per_zone"] = st.get("c2p_iterations", 0) / zones
-        rec["c2p_fallback_fraction"] = st.get("c2p_fallbacks", 0) / zones
-        rec["hlld_to_hllc_fallbacks"] = st.get("hlld_failure", 0)
-        rec["hlld_solves"] = st.get("hlld_success", 0) + st.get("hlld_failure", 0)
-        if st.get("fp64_redo", 0) or st.get("fp64_redo_overflow", 0):
-            rec["fp64_redo_interfaces"] = st.get("fp64_redo", 0)
-            rec["fp64_redo_overflow"] = st.get("fp64_redo_overflow", 0)
-        if name == "learned":
-            g = max(st.get("mlp_guesses", 0), 1)
-            rec["mlp_fallback_fraction"] = st.get("mlp_fallbacks", 0) / g
-        forms_out[name] = rec
+    c5 = None
+    if args.c5 and w.name == "c2":
+        log("[bench] c5 record")
+        w5 = workloads.replace(workloads.C5, precision="fp64", c2p="plain")
+        c5_forms, _ = forms_record(w5, forms, args.c5_steps, 1, w5.steps, args.model)
+        c5 = {"workload": f"c5: {w5.N}^3 periodic RMHD turbulence, 4-velocity rms {w5.v_rms}, Bz = {w5.Bz}, "
+                          f"{w5.extrap.upper()}/{w5.riemann.upper()}/{w5.advance.upper()}",
+              "grid": w5.N, "timed_steps": args.c5_steps, "warmup": 1, "forms": c5_forms}
 
     ws = None
     if args.weno_split_grid:
@@ -454,6 +555,8 @@ def main():
         "forms": forms_out,
         "roofline": roofline,
     }
+    if c5:
+        out["c5"] = c5
     if ws:
         out["weno_split"] = ws
     if pm:

@@ -10,16 +10,19 @@
  *   F[3] face fluxes per axis (double, or float in the mixed form)
  *   A    RK4 accumulator (rk4 only)
  *
- * One step (luaC_Mara_advance, Mara/mara.cpp:297-341) is, per RK stage:
- *   k_cons_to_prim    per interior zone: c2p cascade, result written to the zone and its
- *                     periodic ghost images (replaces Synchronize + per-zone c2p)
+ * One step (luaC_Mara_advance, Mara/mara.cpp:297-341) is:
+ *   k_p2c_c2p         per interior zone: PrimToCons of the committed state and the first
+ *                     c2p pass, its result written to the zone and its periodic ghost
+ *                     images (replaces Synchronize + per-zone c2p)
+ * then per RK stage:
  *   k_flux<DIM> x 3   per interface over the reference's flat sweep range
  *                     (Mara/plm-split.cpp:48): reconstruct 8 fields, Riemann solve,
- *                     PCM retry, max |lambda|
- *   k_update          per interior zone: constrained-transport EMF averages
+ *                     PCM retry, max |lambda| (for HLLD: k_flux_hlld_fast + k_flux_redo)
+ *   k_update_c2p      per interior zone: constrained-transport EMF averages
  *                     (Mara/rmhd.cpp:293-337), flux divergence (plm-split.cpp:117-119) and
- *                     the RK stage combination (Mara/runge-kutta.hpp), all in FP64
- * followed by a final k_cons_to_prim; the state is committed only if no zone failed.
+ *                     the RK stage combination (Mara/runge-kutta.hpp), all in FP64, then
+ *                     the zone's next c2p pass from the new U in registers
+ * The state is committed only if no zone failed.
  */
 #include <hip/hip_runtime.h>
 
@@ -52,8 +55,6 @@ template <typename R> constexpr int kFluxBlock = sizeof(R) == 4 ? 256 : 128;
 constexpr int kFastFluxWaves64 = 2;
 constexpr int kFastFluxWaves32 = 5;
 constexpr int kRedoBlocks = 1024;    // grid of k_flux_redo (a grid-stride loop over its queue)
-// learned c2p: network and polish in one kernel (true) or as two passes over the grid
-constexpr bool kFusedLearnedC2p = true;
 
 /* Device-side counters of one step; reduced per block, then atomically. */
 struct DevStats {
@@ -229,26 +230,6 @@ __global__ void k_field_in(const double *__restrict__ in, double *__restrict__ p
       for (int c = 0; c < nk; ++c) plane[((long)li[a] * g.n[1] + lj[b]) * g.n[2] + lk[c]] = v;
 }
 
-/* GodunovOperator::PrimToCons over the interior (Mara/hydro.cpp:112-123); U0 and U */
-__global__ void k_prim_to_cons(const double *__restrict__ P, double *__restrict__ U0,
-                               double *__restrict__ U, GridDesc g, double gamma)
-{
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  if (t >= ni) return;
-  const long z = interior_zone(g, t);
-  const long T = g.T;
-  double p[8], u[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) p[q] = P[q * T + z];
-  prim_to_cons(p, u, gamma - 1.0);
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    U0[q * T + z] = u[q];
-    U[q * T + z] = u[q];
-  }
-}
-
 /* Writes the 8 fields of interior zone (i,j,k) to the zone and to each periodic ghost
  * image of it: after this every ghost holds its image, which is what
  * SimpleCartesianDomain::Synchronize produces (Mara/simple-cart.cpp:81-170). Returns the
@@ -291,238 +272,83 @@ __global__ void k_conserved_after_reset(const double *__restrict__ P, double *__
   for (int q = 0; q < 8; ++q) Uw[q * T + z] = u[q];
 }
 
-/* Per-zone conserved->primitive (GodunovOperator::ConsToPrim, Mara/hydro.cpp:125-202)
- * with the emergency reset of :165-196. `first` = the previous primitive state is the
- * zero-filled PrimitiveArray (first RK stage). LEARNED: each zone first runs anton2dzw
- * from the network's Newton start (k_mlp_start); a zone whose solve fails runs the plain
- * cascade. */
-template <bool LEARNED>
-__global__ __launch_bounds__(kBlock, 5) void k_cons_to_prim(const double *__restrict__ U,
-                                                         double *__restrict__ P, GridDesc g,
-                                                         double gamma, double pfloor, int first,
-                                                         int ereset, double preset, double dreset,
-                                                         DevStats *st, int pass,
-                                                         const double2 *__restrict__ start,
-                                                         double *__restrict__ Uw)
-{
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  constexpr int NC = LEARNED ? 6 : 4;
-  unsigned long long cnt[NC] = {};
-  if (t < ni) {
-    const long T = g.T;
-    const int k = (int)(t % g.N[2]) + g.Ng;
-    const int j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
-    const int i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
-    const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
-    double u[8], p[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) u[q] = U[q * T + z];
-    int iters = 0, which = 0, err = 1, used = 0;
-    if constexpr (LEARNED) {
-      err = check_cons(u);
-      if (!err) {
-        C2PState s;
-        c2p_new_state(s, u, gamma, pfloor);
-        const double2 zw = start[t];
-        err = c2p_anton2dzw(s, zw.x, zw.y, p, iters);
-        used = err == 0;
-      }
-    }
-    if (err) {
-      // the previous primitive state (the cascade's second guess) is read only here
-#pragma unroll
-      for (int q = 0; q < 8; ++q) p[q] = first ? 0.0 : P[q * T + z];
-      err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
-    }
-    int failed = err != 0;
-    if (failed && ereset) {
-      p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
-      p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
-      failed = 0;
-    }
-    const int mult = scatter_images(g, P, T, i, j, k, p);
-    if (Uw) {
-      scatter_images(g, Uw, T, i, j, k, u);
-      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
-    }
-    cnt[0] = 1;
-    cnt[1] = (unsigned long long)iters;
-    cnt[2] = LEARNED ? !used : which != 0;
-    cnt[3] = failed ? (unsigned long long)mult : 0ull;
-    if constexpr (LEARNED) {
-      cnt[NC - 2] = 1;
-      cnt[NC - 1] = !used;
-    }
-  }
-  if constexpr (LEARNED) {
-    unsigned long long *const dst[6] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
-                                         &st->stage_failed[pass], &st->mlp_guesses, &st->mlp_fallbacks };
-    block_add<6>(cnt, dst);
-  }
-  else {
-    unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
-                                         &st->stage_failed[pass] };
-    block_add<4>(cnt, dst);
-  }
-}
-
 /* Newton start (Z, W) of the anton2dzw solve from the network's raw outputs: W = exp(y0)
  * clamped to [1, 1e6], p = D exp(y1), rho = D / W, h = 1 + Gamma/(Gamma-1) p / rho,
- * Z = rho h W^2. */
+ * Z = rho h W^2. The exponentials are FP32 (the network's outputs carry a few BF16 digits:
+ * the start only needs to lie in Newton's basin). */
 __device__ __forceinline__ void learned_start(double D, float y0, float y1, const MlpDev &net, double gamma,
                                               double &Zg, double &Wg)
 {
-  double W = exp((double)(y0 * net.ystd[0] + net.ymean[0]));
+  double W = (double)__expf(y0 * net.ystd[0] + net.ymean[0]);
   W = W > 1.0 ? (W < 1e6 ? W : 1e6) : 1.0;
-  const double pg = D * exp((double)(y1 * net.ystd[1] + net.ymean[1]));
-  const double rg = D / W;
-  const double hg = 1.0 + gamma / (gamma - 1.0) * pg / rg;
+  const double pg = D * (double)__expf(y1 * net.ystd[1] + net.ymean[1]);
+  const double iW = rcp(W);
+  const double rg = D * iW;
+  const double hg = 1.0 + gamma * rcp(gamma - 1.0) * pg * rcp(rg);
   Zg = rg * hg * W * W;
   Wg = W;
 }
 
-/* The learned c2p in one pass: per 256-zone chunk, the network guess (one wave per 64
- * zones on the matrix cores, weights staged in LDS once per block), the FP64 anton2dzw
- * polish from it, and the plain cascade for zones whose polish fails. Waves of a CU
- * interleave the network's VALU/MFMA work with other waves' FP64 Newton latency, and U
- * is read once: 1.62 ms per launch at 256^3 against 1.98 for the two-pass form. Two
- * waves per SIMD keep it free of spills (193 VGPRs). */
-__global__ __launch_bounds__(kBlock, 2) void k_cons_to_prim_fused(const double *__restrict__ U,
-                                                                  double *__restrict__ P, GridDesc g,
-                                                                  double gamma, double pfloor, int first,
-                                                                  int ereset, double preset, double dreset,
-                                                                  DevStats *st, int pass, MlpDev net,
-                                                                  double *__restrict__ Uw)
+/* The network in LDS: forward fragments and biases, staged once per block, and per-wave
+ * scratch for the layer-1 input rows and the outputs. */
+struct MlpLds {
+  uint16_t wf[MLP_NFWD * MLP_FRAG];
+  float bias[3 * 64 + 2];
+  uint16_t xin[kBlock / 64][64 * MLP_KIN];
+  float yout[kBlock / 64][128];
+};
+
+/* Every thread of the block calls this, then the block synchronises. */
+__device__ __forceinline__ void stage_net(MlpLds &L, const MlpDev &net)
 {
-  __shared__ __align__(16) uint16_t wf[MLP_NFWD * MLP_FRAG];
-  __shared__ __align__(16) float bias[3 * 64 + 2];
-  __shared__ __align__(16) uint16_t xin[kBlock / 64][64 * MLP_KIN];
-  __shared__ float yout[kBlock / 64][128];
   for (int i = threadIdx.x; i < MLP_NFWD * MLP_FRAG / 4; i += kBlock)
-    reinterpret_cast<uint2 *>(wf)[i] = reinterpret_cast<const uint2 *>(net.fwd)[i];
+    reinterpret_cast<uint2 *>(L.wf)[i] = reinterpret_cast<const uint2 *>(net.fwd)[i];
   for (int i = threadIdx.x; i < 3 * 64 + 2; i += kBlock)
-    bias[i] = i < 64 ? net.b1[i] : i < 128 ? net.b2[i - 64] : i < 192 ? net.b3[i - 128] : net.b4[i - 192];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  const long T = g.T;
-  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
-  for (long base = (long)blockIdx.x * kBlock; base < ni; base += (long)gridDim.x * kBlock) {
-    const long t = base + threadIdx.x;
-    const bool valid = t < ni;
-    int i = 0, j = 0, k = 0;
-    long z = 0;
-    double u[8];
-    if (valid) {
-      k = (int)(t % g.N[2]) + g.Ng;
-      j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
-      i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
-      z = ((long)i * g.n[1] + j) * g.n[2] + k;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) u[q] = U[q * T + z];
-    }
-    else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) u[q] = q < 2 ? 1.0 : 0.0;
-    }
-    float x[MLP_IN];
-    mlp_features(u, gamma, x);
-#pragma unroll
-    for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) / net.xstd[q];
-    mlp_input_row(x, &xin[wave][lane * MLP_KIN]);
-    __syncthreads();
-    mlp_forward_wave(xin[wave], wf, bias, bias + 64, bias + 128, bias + 192, yout[wave], nullptr);
-    __syncthreads();
-    const float y0 = yout[wave][lane * 2 + 0], y1 = yout[wave][lane * 2 + 1];
-    if (valid) {
-      double p[8];
-      int iters = 0, which = 0, used = 0;
-      int err = check_cons(u);
-      if (!err) {
-        C2PState s;
-        c2p_new_state(s, u, gamma, pfloor);
-        double Zg, Wg;
-        learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
-        err = c2p_anton2dzw(s, Zg, Wg, p, iters);
-        used = err == 0;
-      }
-      if (err) {
-#pragma unroll
-        for (int q = 0; q < 8; ++q) p[q] = first ? 0.0 : P[q * T + z];
-        err = cons_to_prim(u, p, gamma, pfloor, !first, iters, which);
-      }
-      int failed = err != 0;
-      if (failed && ereset) {
-        p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
-        p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
-        failed = 0;
-      }
-      const int mult = scatter_images(g, P, T, i, j, k, p);
-      if (Uw) {
-      scatter_images(g, Uw, T, i, j, k, u);
-      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
-    }
-      cnt[0] += 1;
-      cnt[1] += (unsigned long long)iters;
-      cnt[2] += used ? 0ull : 1ull;
-      cnt[3] += failed ? (unsigned long long)mult : 0ull;
-      cnt[4] += 1;
-      cnt[5] += used ? 0ull : 1ull;
-    }
-  }
-  unsigned long long *const dst[6] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
-                                       &st->stage_failed[pass], &st->mlp_guesses, &st->mlp_fallbacks };
-  block_add<6>(cnt, dst);
+    L.bias[i] = i < 64 ? net.b1[i] : i < 128 ? net.b2[i - 64] : i < 192 ? net.b3[i - 128] : net.b4[i - 192];
 }
 
-/* The learned c2p's network pass over the interior: one wave per 64 zones on the matrix
- * cores, writing each zone's Newton start (Z, W) (interior index order). Blocks loop over
- * 256-zone chunks so the weights are staged into LDS once per block. */
-__global__ __launch_bounds__(kBlock, 4) void k_mlp_start(const double *__restrict__ U, GridDesc g, double gamma,
-                                                      MlpDev net, double2 *__restrict__ start)
+/* The network's Newton start for each lane's conserved state u. Every lane of the wave
+ * calls it together (a lane without a zone passes a placeholder state); one wave evaluates
+ * its 64 states on the matrix cores. */
+__device__ __forceinline__ void learned_guess(MlpLds &L, const MlpDev &net, const double *u, double gamma,
+                                              double &Zg, double &Wg)
 {
-  __shared__ __align__(16) uint16_t wf[MLP_NFWD * MLP_FRAG];
-  __shared__ __align__(16) float bias[3 * 64 + 2];
-  __shared__ __align__(16) uint16_t xin[kBlock / 64][64 * MLP_KIN];
-  __shared__ float yout[kBlock / 64][128];
-  for (int i = threadIdx.x; i < MLP_NFWD * MLP_FRAG / 4; i += kBlock)
-    reinterpret_cast<uint2 *>(wf)[i] = reinterpret_cast<const uint2 *>(net.fwd)[i];
-  for (int i = threadIdx.x; i < 3 * 64 + 2; i += kBlock)
-    bias[i] = i < 64 ? net.b1[i] : i < 128 ? net.b2[i - 64] : i < 192 ? net.b3[i - 128] : net.b4[i - 192];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  const long T = g.T;
-  for (long base = (long)blockIdx.x * kBlock; base < ni; base += (long)gridDim.x * kBlock) {
-    const long t = base + threadIdx.x;
-    const bool valid = t < ni;
-    double u[8];
-    if (valid) {
-      const long k = t % g.N[2] + g.Ng;
-      const long j = (t / g.N[2]) % g.N[1] + g.Ng;
-      const long i = t / ((long)g.N[2] * g.N[1]) + g.Ng;
-      const long z = (i * g.n[1] + j) * g.n[2] + k;
+  float x[MLP_IN];
+  mlp_features(u, gamma, x);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) u[q] = U[q * T + z];
-    }
-    else {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) u[q] = q < 2 ? 1.0 : 0.0;
-    }
-    float x[MLP_IN];
-    mlp_features(u, gamma, x);
-#pragma unroll
-    for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) / net.xstd[q];
-    mlp_input_row(x, &xin[wave][lane * MLP_KIN]);
-    __syncthreads();
-    mlp_forward_wave(xin[wave], wf, bias, bias + 64, bias + 128, bias + 192, yout[wave], nullptr);
-    __syncthreads();
-    const float y0 = yout[wave][lane * 2 + 0], y1 = yout[wave][lane * 2 + 1];
-    if (valid) {
-      double Zg, Wg;
-      learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
-      start[t] = make_double2(Zg, Wg);
-    }
+  for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) * net.xinv[q];
+  mlp_input_row(x, &L.xin[wave][lane * MLP_KIN]);
+  wave_lds_sync();
+  mlp_forward_wave(L.xin[wave], L.wf, L.bias, L.bias + 64, L.bias + 128, L.bias + 192, L.yout[wave], nullptr);
+  wave_lds_sync();
+  const float y0 = L.yout[wave][lane * 2 + 0], y1 = L.yout[wave][lane * 2 + 1];
+  wave_lds_sync();   // the wave's next call rewrites xin and yout
+  learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
+}
+
+/* The conserved-to-primitive solve of one zone in the learned form: anton2dzw from the
+ * network's start; where that fails, the plain cascade (GodunovOperator::ConsToPrim,
+ * Mara/rmhd.cpp:59-131) with the previous primitive state, which p holds on entry when
+ * prev_valid and which is read (prev()) only then. Returns the cascade's error. */
+template <typename Prev>
+__device__ __forceinline__ int learned_c2p(const double *u, double *p, double Zg, double Wg, double gamma,
+                                           double pfloor, bool prev_valid, Prev prev, int &iters, int &used)
+{
+  int which = 0;
+  int err = check_cons(u);
+  used = 0;
+  if (!err) {
+    C2PState s;
+    c2p_new_state(s, u, gamma, pfloor);
+    err = c2p_anton2dzw(s, Zg, Wg, p, iters);
+    used = err == 0;
   }
+  if (err) {
+    prev(p);
+    err = cons_to_prim(u, p, gamma, pfloor, prev_valid, iters, which);
+  }
+  return err;
 }
 
 __global__ __launch_bounds__(kBlock) void kb_cons_to_prim_learned(long n, const double *__restrict__ U,
@@ -544,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void kb_cons_to_prim_learned(long n, const 
   float x[MLP_IN];
   mlp_features(u, gamma, x);
 #pragma unroll
-  for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) / net.xstd[q];
+  for (int q = 0; q < MLP_IN; ++q) x[q] = (x[q] - net.xmean[q]) * net.xinv[q];
   mlp_input_row(x, &xin[wave][lane * MLP_KIN]);
   __syncthreads();
   mlp_forward_wave(xin[wave], net.fwd, net.b1, net.b2, net.b3, net.b4, yout[wave], nullptr);
@@ -711,6 +537,62 @@ __global__ __launch_bounds__(kBlock, sizeof(R) == 4 ? 3 : 2) void k_flux(const d
   block_add<7>(cnt, dst);
 }
 
+/* Block order of the HLLD sweeps. The interfaces of a sweep along an axis of stride S are
+ * z = a S + r over planes a in [2, T/S - 3) and offsets r in [0, S) (the reference's flat
+ * range [2S, T-3S)); an interface's stencil lies at a-3..a+2 in the same column r.
+ *   - The hardware hands blocks to the eight XCDs (each with its own L2) round-robin.
+ *     Logical block L is given to XCD L / (nb/8): each XCD works through one contiguous
+ *     eighth of the logical order, so the planes a stencil re-reads are in its own L2.
+ *   - Along x (S = one i-plane, 4.4 MB of primitives at 256^3) the logical order is tiled:
+ *     columns of kSweepChunk tiles of r are swept over every plane before the next
+ *     column starts, so the six planes a stencil spans are a few tile-rows apart, not
+ *     whole planes. Along y and z the flat order already keeps them within a few blocks.
+ * The interfaces, and so every result, are those of the flat order; only their order on
+ * the chip changes. */
+constexpr long kSweepChunk = 8;
+
+/* Number of blocks of BLK threads of the sweep along DIM. */
+template <int DIM>
+long sweep_blocks(const GridDesc &g, long BLK)
+{
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  if (DIM == 1) return (g.T / S - 5) * ((S + BLK - 1) / BLK);
+  return (g.T - 5 * S + BLK - 1) / BLK;
+}
+
+/* The calling thread's interface z of the sweep; false where it has none. */
+template <int DIM>
+__device__ __forceinline__ bool sweep_interface(const GridDesc &g, long &z)
+{
+  const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
+  const long BLK = blockDim.x, nb = gridDim.x, b = blockIdx.x;
+  const long q = nb >> 3;
+  const long L = b < 8 * q ? (b & 7) * q + (b >> 3) : b;
+  if constexpr (DIM == 1) {
+    const long ntiles = (S + BLK - 1) / BLK;
+    const long nA = g.T / S - 5;
+    const long full = ntiles / kSweepChunk, per_full = nA * kSweepChunk;
+    long a, tile;
+    if (L < full * per_full) {
+      const long rem = L % per_full;
+      a = rem / kSweepChunk;
+      tile = (L / per_full) * kSweepChunk + rem % kSweepChunk;
+    }
+    else {
+      const long width = ntiles - full * kSweepChunk, rem = L - full * per_full;
+      a = rem / width;
+      tile = full * kSweepChunk + rem % width;
+    }
+    const long r = tile * BLK + threadIdx.x;
+    z = (2 + a) * S + r;
+    return r < S;
+  }
+  else {
+    z = 2 * S + L * BLK + threadIdx.x;
+    return z < g.T - 3 * S;
+  }
+}
+
 /* HLLD sweep along DIM, common path only (interface_flux_fast): interfaces that need
  * the c2p cascade, a later Newton attempt, the small-normal-field branch, the HLLC
  * fallback or the retry are queued for k_flux_redo, which computes them with the full
@@ -725,11 +607,12 @@ void k_flux_hlld_fast(const double *__restrict__ P, R *__restrict__ Fout, GridDe
 {
   const long T = g.T;
   const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
-  const long z = 2 * S + (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long z;
+  const bool has = sweep_interface<DIM>(g, z);
   double lam_out = 0.0;
   unsigned long long cnt[3] = {0, 0, 0};
   bool defer = false;
-  if (z < T - 3 * S) {
+  if (has) {
     R lam = R(0);
     FluxCounters fc = {0, 0, 0, 0};
     R F[8];
@@ -1277,116 +1160,246 @@ __device__ __forceinline__ void update_zone(const FT *__restrict__ Fx, const FT 
   }
 }
 
-template <typename FT>
-__global__ __launch_bounds__(kBlock) void k_update(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
-                                                   const FT *__restrict__ Fz,
-                                                   const double *__restrict__ U0, double *__restrict__ U,
-                                                   double *__restrict__ A, GridDesc g, double dt, int kind)
+/* The zone's result of a conserved-to-primitive pass: the emergency reset of
+ * Mara/hydro.cpp:165-196 where the cascade failed and ereset is on, the primitive state
+ * (and with Uw the conserved state) written to the zone and its periodic images, and the
+ * pass's counters: zones, Newton iterations, fallbacks, failed zones (with the zone's
+ * multiplicity, as Mara's all-zone loops count them) and, in the learned form, network
+ * starts and their fallbacks. */
+template <int NC>
+__device__ __forceinline__ void finish_c2p(const GridDesc &g, double *__restrict__ P, double *__restrict__ Uw,
+                                           int i, int j, int k, const double *u, double *p, int err, int ereset,
+                                           double preset, double dreset, DevStats *st, int pass, int iters,
+                                           bool fallback, unsigned long long (&cnt)[NC])
 {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  if (t >= ni) return;
-  double unew[8];
-  update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, interior_zone(g, t), dt, kind, unew);
+  int failed = err != 0;
+  if (failed && ereset) {
+    p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
+    p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
+    failed = 0;
+  }
+  const int mult = scatter_images(g, P, g.T, i, j, k, p);
+  if (Uw) {
+    scatter_images(g, Uw, g.T, i, j, k, u);
+    if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
+  }
+  cnt[0] += 1;
+  cnt[1] += (unsigned long long)iters;
+  cnt[2] += fallback ? 1ull : 0ull;
+  cnt[3] += failed ? (unsigned long long)mult : 0ull;
+  if constexpr (NC == 6) {
+    cnt[4] += 1;
+    cnt[5] += fallback ? 1ull : 0ull;
+  }
+}
+
+template <int NC>
+__device__ __forceinline__ void c2p_counters_out(const unsigned long long (&cnt)[NC], DevStats *st, int pass)
+{
+  if constexpr (NC == 6) {
+    unsigned long long *const dst[6] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                         &st->stage_failed[pass], &st->mlp_guesses, &st->mlp_fallbacks };
+    block_add<6>(cnt, dst);
+  }
+  else {
+    unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
+                                         &st->stage_failed[pass] };
+    block_add<4>(cnt, dst);
+  }
+}
+
+/* Grid of the learned kernels: they loop over 256-zone chunks, so the network is staged
+ * into LDS once per block. */
+constexpr unsigned kLearnedBlocks = 4096;
+
+/* The conserved-to-primitive pass's start: the plain cascade (C2P_CASCADE), the network on
+ * the zone's current U (C2P_NET), or the network's start of the step's first pass, stored
+ * per zone and reused (C2P_STORED). */
+enum { C2P_CASCADE = 0, C2P_NET = 1, C2P_STORED = 2 };
+// the learned form's stage passes: the network on every pass (false) or its start of the
+// step's first pass reused (true)
+constexpr bool kLearnedStartOncePerStep = true;
+
+/* Interior zone of the calling thread for the one-zone-per-thread update kernels, in the
+ * order of the x sweep (sweep_interface): the interior's (j, k) planes cut into tiles of
+ * blockDim zones, columns of kSweepChunk tiles swept over every plane i before the next
+ * column, logical blocks in XCD-contiguous ranges. The fluxes a zone's CT and divergence
+ * read at i - 1 are then a few blocks back, in the same L2. Returns the interior flat
+ * index t (k fastest), or -1 where the thread has no zone. */
+constexpr bool kUpdateTiled = true;
+
+__device__ __forceinline__ long update_zone_index(const GridDesc &g)
+{
+  const long BLK = blockDim.x, nb = gridDim.x, b = blockIdx.x;
+  const long q = nb >> 3;
+  const long L = b < 8 * q ? (b & 7) * q + (b >> 3) : b;
+  const long NP = (long)g.N[1] * g.N[2], N0 = g.N[0];
+  const long ntiles = (NP + BLK - 1) / BLK;
+  const long full = ntiles / kSweepChunk, per_full = N0 * kSweepChunk;
+  long i, tile;
+  if (L < full * per_full) {
+    const long rem = L % per_full;
+    i = rem / kSweepChunk;
+    tile = (L / per_full) * kSweepChunk + rem % kSweepChunk;
+  }
+  else {
+    const long width = ntiles - full * kSweepChunk, rem = L - full * per_full;
+    i = rem / width;
+    tile = full * kSweepChunk + rem % width;
+  }
+  const long r = tile * BLK + threadIdx.x;
+  return r < NP ? i * NP + r : -1;
+}
+
+/* Grid of a one-zone-per-thread update kernel (with update_zone_index). */
+unsigned update_blocks(const GridDesc &g)
+{
+  const long NP = (long)g.N[1] * g.N[2];
+  return (unsigned)(g.N[0] * ((NP + kBlock - 1) / kBlock));
 }
 
 /* The stage update followed at once by the next conserved-to-primitive pass of the zone
- * (plain c2p, not the first pass): k_update and k_cons_to_prim<false> in one kernel, the new
- * U taken from registers instead of read back. Same arithmetic, same counters; with Uw
- * (weno-split, plm-muscl) U's ghost images are written as k_cons_to_prim writes them. */
-template <typename FT>
-__global__ __launch_bounds__(kBlock, 3) void k_update_c2p(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
-                                                          const FT *__restrict__ Fz,
-                                                          const double *__restrict__ U0, double *__restrict__ U,
-                                                          double *__restrict__ A, double *__restrict__ P, GridDesc g,
-                                                          double dt, int kind, double gamma, double pfloor, int ereset,
-                                                          double preset, double dreset, DevStats *st, int pass,
-                                                          double *__restrict__ Uw)
+ * (not the first pass): CT, the flux divergence and the RK combination, then the new U
+ * from registers into the inversion (the plain cascade, or with LEARNED the network's
+ * start and the cascade as fallback). With Uw (weno-split, plm-muscl) U's ghost images are
+ * written too. */
+template <typename FT, int MODE>
+__global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
+    const FT *__restrict__ Fx, const FT *__restrict__ Fy, const FT *__restrict__ Fz,
+    const double *__restrict__ U0, double *__restrict__ U, double *__restrict__ A, double *__restrict__ P,
+    GridDesc g, double dt, int kind, double gamma, double pfloor, int ereset, double preset, double dreset,
+    DevStats *st, int pass, double *__restrict__ Uw, MlpDev net, const double2 *__restrict__ start)
 {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  unsigned long long cnt[4] = {0, 0, 0, 0};
-  if (t < ni) {
-    const long T = g.T;
-    const int k = (int)(t % g.N[2]) + g.Ng;
-    const int j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
-    const int i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
-    const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
-    double u[8], p[8];
-    update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, z, dt, kind, u);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) p[q] = P[q * T + z];
-    int iters = 0, which = 0;
-    const int err = cons_to_prim(u, p, gamma, pfloor, true, iters, which);
-    int failed = err != 0;
-    if (failed && ereset) {
-      p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
-      p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
-      failed = 0;
-    }
-    const int mult = scatter_images(g, P, T, i, j, k, p);
-    if (Uw) {
-      scatter_images(g, Uw, T, i, j, k, u);
-      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
-    }
-    cnt[0] = 1;
-    cnt[1] = (unsigned long long)iters;
-    cnt[2] = which != 0;
-    cnt[3] = failed ? (unsigned long long)mult : 0ull;
+  constexpr bool LEARNED = MODE == C2P_NET;
+  constexpr int NC = MODE == C2P_CASCADE ? 4 : 6;
+  MlpLds *L = nullptr;
+  if constexpr (LEARNED) {
+    __shared__ __align__(16) MlpLds lds;
+    L = &lds;
+    stage_net(lds, net);
+    __syncthreads();
   }
-  unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
-                                       &st->stage_failed[pass] };
-  block_add<4>(cnt, dst);
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  const long T = g.T;
+  unsigned long long cnt[NC] = {};
+  const long step = LEARNED ? (long)gridDim.x * kBlock : ni;
+  for (long base = (long)blockIdx.x * kBlock; base < ni; base += step) {
+    const long t = (!LEARNED && kUpdateTiled) ? update_zone_index(g) : base + threadIdx.x;
+    const bool valid = t >= 0 && t < ni;
+    int i = 0, j = 0, k = 0;
+    long z = 0;
+    double u[8], p[8];
+    if (valid) {
+      k = (int)(t % g.N[2]) + g.Ng;
+      j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+      i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+      z = ((long)i * g.n[1] + j) * g.n[2] + k;
+      update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, z, dt, kind, u);
+    }
+    else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) u[q] = q < 2 ? 1.0 : 0.0;
+    }
+    int iters = 0, err;
+    bool fallback;
+    if constexpr (MODE != C2P_CASCADE) {
+      double Zg, Wg;
+      if constexpr (LEARNED) learned_guess(*L, net, u, gamma, Zg, Wg);
+      if (!valid) continue;
+      if constexpr (MODE == C2P_STORED) {
+        const double2 zw = start[t];
+        Zg = zw.x;
+        Wg = zw.y;
+      }
+      int used;
+      err = learned_c2p(u, p, Zg, Wg, gamma, pfloor, true, [&](double *q) {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) q[f] = P[f * T + z];
+      }, iters, used);
+      fallback = !used;
+    }
+    else {
+      if (!valid) continue;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) p[q] = P[q * T + z];
+      int which = 0;
+      err = cons_to_prim(u, p, gamma, pfloor, true, iters, which);
+      fallback = which != 0;
+    }
+    finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
+  }
+  c2p_counters_out<NC>(cnt, st, pass);
 }
 
-/* The start of a step with the plain c2p: GodunovOperator::PrimToCons of the committed state
- * (U0 and U, as k_prim_to_cons) and the first conserved-to-primitive pass of the zone (the
- * zero-filled previous state, as k_cons_to_prim<false> with first = 1) in one kernel. */
-__global__ __launch_bounds__(kBlock, 5) void k_p2c_c2p(const double *__restrict__ Pc, double *__restrict__ U0,
-                                                      double *__restrict__ U, double *__restrict__ P, GridDesc g,
-                                                      double gamma, double pfloor, int ereset, double preset,
-                                                      double dreset, DevStats *st, int pass, double *__restrict__ Uw)
+/* The start of a step: GodunovOperator::PrimToCons of the committed state (U0 and U) and
+ * the first conserved-to-primitive pass of the zone, whose previous state is the
+ * zero-filled PrimitiveArray (Mara/mara.cpp:321), in one kernel; LEARNED as above. */
+template <bool LEARNED>
+__global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const double *__restrict__ Pc,
+                                                                    double *__restrict__ U0, double *__restrict__ U,
+                                                                    double *__restrict__ P, GridDesc g, double gamma,
+                                                                    double pfloor, int ereset, double preset,
+                                                                    double dreset, DevStats *st, int pass,
+                                                                    double *__restrict__ Uw, MlpDev net,
+                                                                    double2 *__restrict__ start)
 {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
-  unsigned long long cnt[4] = {0, 0, 0, 0};
-  if (t < ni) {
-    const long T = g.T;
-    const int k = (int)(t % g.N[2]) + g.Ng;
-    const int j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
-    const int i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
-    const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
-    double u[8], p[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) p[q] = Pc[q * T + z];
-    prim_to_cons(p, u, gamma - 1.0);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      U0[q * T + z] = u[q];
-      U[q * T + z] = u[q];
-      p[q] = 0.0;
-    }
-    int iters = 0, which = 0;
-    const int err = cons_to_prim(u, p, gamma, pfloor, false, iters, which);
-    int failed = err != 0;
-    if (failed && ereset) {
-      p[0] = dreset; p[1] = preset; p[2] = 0.0; p[3] = 0.0; p[4] = 0.0;
-      p[5] = u[5]; p[6] = u[6]; p[7] = u[7];
-      failed = 0;
-    }
-    const int mult = scatter_images(g, P, T, i, j, k, p);
-    if (Uw) {
-      scatter_images(g, Uw, T, i, j, k, u);
-      if (err != 0 && ereset) atomicAdd(&st->stage_reset[pass], 1ull);
-    }
-    cnt[0] = 1;
-    cnt[1] = (unsigned long long)iters;
-    cnt[2] = which != 0;
-    cnt[3] = failed ? (unsigned long long)mult : 0ull;
+  constexpr int NC = LEARNED ? 6 : 4;
+  MlpLds *L = nullptr;
+  if constexpr (LEARNED) {
+    __shared__ __align__(16) MlpLds lds;
+    L = &lds;
+    stage_net(lds, net);
+    __syncthreads();
   }
-  unsigned long long *const dst[4] = { &st->c2p_zones, &st->c2p_iters, &st->c2p_fallbacks,
-                                       &st->stage_failed[pass] };
-  block_add<4>(cnt, dst);
+  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
+  const long T = g.T;
+  unsigned long long cnt[NC] = {};
+  const long step = LEARNED ? (long)gridDim.x * kBlock : ni;
+  for (long base = (long)blockIdx.x * kBlock; base < ni; base += step) {
+    const long t = base + threadIdx.x;
+    const bool valid = t < ni;
+    int i = 0, j = 0, k = 0;
+    double u[8], p[8];
+    if (valid) {
+      k = (int)(t % g.N[2]) + g.Ng;
+      j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
+      i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
+      const long z = ((long)i * g.n[1] + j) * g.n[2] + k;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) p[q] = Pc[q * T + z];
+      prim_to_cons(p, u, gamma - 1.0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        U0[q * T + z] = u[q];
+        U[q * T + z] = u[q];
+      }
+    }
+    else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) u[q] = q < 2 ? 1.0 : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) p[q] = 0.0;
+    int iters = 0, err;
+    bool fallback;
+    if constexpr (LEARNED) {
+      double Zg, Wg;
+      learned_guess(*L, net, u, gamma, Zg, Wg);
+      if (!valid) continue;
+      if (start) start[t] = make_double2(Zg, Wg);
+      int used;
+      err = learned_c2p(u, p, Zg, Wg, gamma, pfloor, false, [](double *) {}, iters, used);
+      fallback = !used;
+    }
+    else {
+      if (!valid) continue;
+      int which = 0;
+      err = cons_to_prim(u, p, gamma, pfloor, false, iters, which);
+      fallback = which != 0;
+    }
+    finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
+  }
+  c2p_counters_out<NC>(cnt, st, pass);
 }
 
 /* luaC_mean_max_divB stencil, Mara/mara.cpp:2298-2314, over its loop range */
@@ -1918,6 +1931,7 @@ int dalloc(DevBuf &b, size_t bytes) { return hipMalloc(&b.p, bytes ? bytes : 1) 
 struct maragpu_ctx {
   maragpu_config cfg;
   GridDesc g;
+  int device = 0;                    // the HIP device current at create
   hipStream_t stream = nullptr;
   double *Pc = nullptr, *Pw = nullptr, *U0 = nullptr, *U = nullptr, *A = nullptr;
   void *F[3] = {nullptr, nullptr, nullptr};
@@ -1930,8 +1944,8 @@ struct maragpu_ctx {
   std::vector<hipEvent_t> ev;
   double flux_ms = 0.0, c2p_ms = 0.0, upd_ms = 0.0;
   int flux_n = 0, c2p_n = 0, upd_n = 0;
-  // learned c2p network (not owned; see maragpu_attach_mlp) and the Newton starts it
-  // writes (one (Z, W) per interior zone)
+  // learned c2p network (not owned; see maragpu_attach_mlp) and the Newton starts of the
+  // step's first pass (one (Z, W) per interior zone), which the stage passes reuse
   const maragpu_mlp *mlp = nullptr;
   double2 *start = nullptr;
   // mixed form with HLLD: the FP64 redo queue of each sweep
@@ -2010,8 +2024,8 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
     // the sweep, then its queued interfaces by the full FP64 solver: in the mixed form the
     // FP32 HLLD failures, in the plain form whatever left the common path
     hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
-    k_flux_hlld_fast<R, DIM><<<(unsigned)((n + kFluxBlock<R> - 1) / kFluxBlock<R>), kFluxBlock<R>, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats,
-                                                                    c->redo);
+    k_flux_hlld_fast<R, DIM><<<(unsigned)sweep_blocks<DIM>(c->g, kFluxBlock<R>), kFluxBlock<R>, 0, c->stream>>>(
+        c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, c->redo);
     k_flux_redo<FT, DIM, RIEMANN_HLLD><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp,
                                                                               c->dstats, c->redo);
     break;
@@ -2082,8 +2096,11 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MARAGPU_E_HIP;
 
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return MARAGPU_E_HIP;
   maragpu_ctx *c = new (std::nothrow) maragpu_ctx;
   if (!c) return MARAGPU_E_NOMEM;
+  c->device = dev;
   c->cfg = *cfg;
   c->g = make_grid(*cfg);
   if (c->g.T >= (1L << 32)) { delete c; return MARAGPU_E_INVALID; }   // 32-bit queue slots
@@ -2100,7 +2117,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (hipMalloc(&c->dstats, sizeof(DevStats)) != hipSuccess) return fail_free(c);
   if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
   if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
-  if (cfg->c2p == MARAGPU_C2P_LEARNED && !kFusedLearnedC2p &&
+  if (cfg->c2p == MARAGPU_C2P_LEARNED && kLearnedStartOncePerStep &&
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(double2)) != hipSuccess)
     return fail_free(c);
   if (cfg->godunov == MARAGPU_WENO_SPLIT) {
@@ -2245,41 +2262,12 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   HIP_CHECK(hipMemsetAsync(c->dstats, 0, sizeof(DevStats), c->stream));
   HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));
   const int ev_start = evi++;
-  // with the plain c2p each update runs together with the c2p pass after it, and the step's
-  // PrimToCons with its first pass
-  const bool fuse = c->cfg.c2p == MARAGPU_C2P_PLAIN;
-  if (!fuse) k_prim_to_cons<<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, g, c->cfg.gamma);
-
-  auto c2p = [&](int first, int pass) -> int {
-    EventPair p = mark(ec);
-    HIP_CHECK(hipEventRecord(c->ev[p.a], c->stream));
-    if (c->cfg.c2p == MARAGPU_C2P_LEARNED) {
-      MlpDev net;
-      if (maragpu_mlp_device_params(c->mlp, &net)) return MARAGPU_E_NOMODEL;
-      const unsigned nb = nblocks(ni) < 4096u ? nblocks(ni) : 4096u;
-      if (kFusedLearnedC2p) {
-        k_cons_to_prim_fused<<<nb, kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor, first,
-                                                           c->cfg.ereset, c->cfg.preset, c->cfg.dreset,
-                                                           c->dstats, pass, net, c->Uw);
-      }
-      else {
-        k_mlp_start<<<nb, kBlock, 0, c->stream>>>(c->U, g, c->cfg.gamma, net, c->start);
-        k_cons_to_prim<true><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
-                                                                    first, c->cfg.ereset, c->cfg.preset,
-                                                                    c->cfg.dreset, c->dstats, pass, c->start, c->Uw);
-      }
-    }
-    else {
-      k_cons_to_prim<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
-                                                                   first, c->cfg.ereset, c->cfg.preset,
-                                                                   c->cfg.dreset, c->dstats, pass, nullptr, c->Uw);
-    }
-    if (c->Uw && c->cfg.ereset)
-      k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
-                                                                      &c->dstats->stage_reset[pass]);
-    HIP_CHECK(hipEventRecord(c->ev[p.b], c->stream));
-    return 0;
-  };
+  // each update runs together with the c2p pass after it, and the step's PrimToCons with
+  // its first pass
+  const bool learned = c->cfg.c2p == MARAGPU_C2P_LEARNED;
+  MlpDev net = {};
+  if (learned && maragpu_mlp_device_params(c->mlp, &net)) return MARAGPU_E_NOMODEL;
+  const unsigned nbl = nblocks(ni) < kLearnedBlocks ? nblocks(ni) : kLearnedBlocks;
   auto sweeps = [&]() {
     for (int d = 1; d <= 3; ++d) {
       EventPair p = mark(ef);
@@ -2309,18 +2297,6 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
       hipEventRecord(c->ev[p.b], c->stream);
     }
   };
-  auto update = [&](int kind) {
-    EventPair p = mark(eu);
-    hipEventRecord(c->ev[p.a], c->stream);
-    if (mixed)
-      k_update<float><<<nblocks(ni), kBlock, 0, c->stream>>>((const float *)c->F[0], (const float *)c->F[1],
-                                                              (const float *)c->F[2], c->U0, c->U, c->A, g, dt, kind);
-    else
-      k_update<double><<<nblocks(ni), kBlock, 0, c->stream>>>((const double *)c->F[0], (const double *)c->F[1],
-                                                               (const double *)c->F[2], c->U0, c->U, c->A, g, dt, kind);
-    hipEventRecord(c->ev[p.b], c->stream);
-  };
-
   std::vector<int> kinds;
   switch (c->cfg.rk) {
   case 1: kinds = { ST_EULER }; break;
@@ -2375,14 +2351,20 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   auto update_c2p = [&](int kind, int pass) {
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
-    if (mixed)
-      k_update_c2p<float><<<nblocks(ni), kBlock, 0, c->stream>>>(
-          (const float *)c->F[0], (const float *)c->F[1], (const float *)c->F[2], c->U0, c->U, c->A, c->Pw, g, dt,
-          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw);
-    else
-      k_update_c2p<double><<<nblocks(ni), kBlock, 0, c->stream>>>(
-          (const double *)c->F[0], (const double *)c->F[1], (const double *)c->F[2], c->U0, c->U, c->A, c->Pw, g, dt,
-          kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw);
+    const unsigned nb = learned ? nbl : (kUpdateTiled ? update_blocks(g) : nblocks(ni));
+#define UPDATE_ARGS(FT) (const FT *)c->F[0], (const FT *)c->F[1], (const FT *)c->F[2], c->U0, c->U, c->A, c->Pw, \
+    g, dt, kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, net, \
+    c->start
+    const unsigned nbu = kUpdateTiled ? update_blocks(g) : nblocks(ni);
+    if (learned && kLearnedStartOncePerStep) {
+      if (mixed) k_update_c2p<float, C2P_STORED><<<nbu, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
+      else k_update_c2p<double, C2P_STORED><<<nbu, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
+    }
+    else if (mixed && learned) k_update_c2p<float, C2P_NET><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
+    else if (mixed) k_update_c2p<float, C2P_CASCADE><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
+    else if (learned) k_update_c2p<double, C2P_NET><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
+    else k_update_c2p<double, C2P_CASCADE><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
+#undef UPDATE_ARGS
     if (c->Uw && c->cfg.ereset)
       k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
                                                                       &c->dstats->stage_reset[pass]);
@@ -2391,9 +2373,14 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   auto p2c_c2p = [&](int pass) {
     EventPair p = mark(ec);
     hipEventRecord(c->ev[p.a], c->stream);
-    k_p2c_c2p<<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
-                                                      c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass,
-                                                      c->Uw);
+    if (learned)
+      k_p2c_c2p<true><<<nbl, kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+                                                     c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass,
+                                                     c->Uw, net, kLearnedStartOncePerStep ? c->start : nullptr);
+    else
+      k_p2c_c2p<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, c->Pw, g, c->cfg.gamma,
+                                                              c->cfg.pfloor, c->cfg.ereset, c->cfg.preset,
+                                                              c->cfg.dreset, c->dstats, pass, c->Uw, net, nullptr);
     if (c->Uw && c->cfg.ereset)
       k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
                                                                       &c->dstats->stage_reset[pass]);
@@ -2401,19 +2388,10 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   };
   int pass = 0;
   for (size_t s = 0; s < kinds.size(); ++s) {
-    if (s == 0 && fuse) p2c_c2p(pass++);
-    else if (s == 0 || !fuse) {
-      int rc = c2p(s == 0, pass++);
-      if (rc) return rc;
-    }
+    if (s == 0) p2c_c2p(pass++);
     if (ctu) ctu_fluxes((int)s);
     else sweeps();
-    if (fuse) update_c2p(kinds[s], pass++);
-    else update(kinds[s]);
-  }
-  if (!fuse) {
-    int rc = c2p(0, pass++);
-    if (rc) return rc;
+    update_c2p(kinds[s], pass++);
   }
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));
@@ -2528,9 +2506,13 @@ int maragpu_measure_divB(maragpu_ctx *c, double *mean, double *max, double *absm
   return MARAGPU_OK;
 }
 
-int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
+}  // extern "C"
+
+namespace {
+/* The measure table of the committed state with the per-block partials and the reduced
+ * row in `part` ((kMeasBlocks + 1) x kMeasQ doubles, every row used written first). */
+int measure_into(maragpu_ctx *c, const double *units, maragpu_measures *out, double *part)
 {
-  if (!c || !out) return MARAGPU_E_INVALID;
   // PhysicalUnits(Length, Mass, Time), Mara/units.cpp:3-6 and :92-141; cgs by default
   // (HydroModule's units, Mara/hydro.cpp:20); constants of Mara/eos.hpp:12-15
   const double cgs[3] = {1.0, 1.0, 1.0};
@@ -2540,16 +2522,12 @@ int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
   const double Erg = (1.0 / units[1]) * std::pow(Centimeter / Second, 2.0);
   const double Gauss = std::sqrt(Erg / std::pow(Centimeter, 3.0) / (4 * M_PI));
   const MeasureUnits mu = measure_units(units);
-  if (!c->dmeas && hipMalloc(&c->dmeas, (size_t)(kMeasBlocks + 1) * kMeasQ * sizeof(double)) != hipSuccess) {
-    c->dmeas = nullptr;
-    return MARAGPU_E_NOMEM;
-  }
   const GridDesc &g = c->g;
   const long ni = (long)g.N[0] * g.N[1] * g.N[2];
   const int nb = (int)std::min<long>(kMeasBlocks, (ni + kBlock - 1) / kBlock);
-  double *res = c->dmeas + (size_t)kMeasBlocks * kMeasQ;
-  k_measure<<<nb, kBlock, 0, c->stream>>>(c->Pc, g, c->cfg.gamma, mu, c->dmeas);
-  k_measure_reduce<<<kMeasQ, kBlock, 0, c->stream>>>(c->dmeas, nb, res);
+  double *res = part + (size_t)kMeasBlocks * kMeasQ;
+  k_measure<<<nb, kBlock, 0, c->stream>>>(c->Pc, g, c->cfg.gamma, mu, part);
+  k_measure_reduce<<<kMeasQ, kBlock, 0, c->stream>>>(part, nb, res);
   double r[kMeasQ];
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(r, res, sizeof(r), hipMemcpyDeviceToHost, c->stream));
@@ -2569,6 +2547,20 @@ int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
   out->mean_max_magnetic_field[1] = (1.0 / Gauss) * r[kMeasSums + 3];
   out->max_lorentz_factor = r[kMeasSums + 4];
   return MARAGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
+{
+  if (!c || !out) return MARAGPU_E_INVALID;
+  if (!c->dmeas && hipMalloc(&c->dmeas, (size_t)(kMeasBlocks + 1) * kMeasQ * sizeof(double)) != hipSuccess) {
+    c->dmeas = nullptr;
+    return MARAGPU_E_NOMEM;
+  }
+  return measure_into(c, units, out, c->dmeas);
 }
 
 namespace {

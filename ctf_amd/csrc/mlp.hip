@@ -28,6 +28,7 @@ namespace {
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + 64 * 6, OFF_W2 = OFF_B1 + 64, OFF_B2 = OFF_W2 + 64 * 64;
 constexpr int OFF_W3 = OFF_B2 + 64, OFF_B3 = OFF_W3 + 64 * 64, OFF_W4 = OFF_B3 + 64;
 constexpr int OFF_B4 = OFF_W4 + 2 * 64, NPARAM = OFF_B4 + 2;
+constexpr uint32_t kSerialVersion = 2;   // the activation is part of the format
 
 #define HIPOK(expr)                                                                  \
   do {                                                                               \
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(256) void k_train_fwd(const float *__restrict__ X, 
   const long src = idx ? idx[s] : s;
   {
     float xs[MLP_IN];
-    for (int k = 0; k < MLP_IN; ++k) xs[k] = (X[src * MLP_IN + k] - nm.xmean[k]) / nm.xstd[k];
+    for (int k = 0; k < MLP_IN; ++k) xs[k] = (X[src * MLP_IN + k] - nm.xmean[k]) * (1.0f / nm.xstd[k]);
     uint16_t *row = &xin[wave][lane * MLP_KIN];
     mlp_input_row(xs, row);
     for (int k = 0; k < MLP_KIN; ++k) H[(long)k * B + s] = row[k];
@@ -185,10 +186,10 @@ __global__ __launch_bounds__(256) void k_train_fwd(const float *__restrict__ X, 
   __syncthreads();
   yout[s * 2 + 0] = ybuf[wave][lane * 2 + 0];
   yout[s * 2 + 1] = ybuf[wave][lane * 2 + 1];
-  // layer inputs H1..H3 = bf16(silu(pre-activation)), [feature][batch]
+  // layer inputs H1..H3 = bf16(hswish(pre-activation)), [feature][batch]
   for (int layer = 0; layer < 3; ++layer)
     for (int f = 0; f < 64; ++f)
-      H[(long)(MLP_KIN + layer * 64 + f) * B + s] = f32_to_bf16(silu(a[(layer * 64 + f) * 64 + lane]));
+      H[(long)(MLP_KIN + layer * 64 + f) * B + s] = f32_to_bf16(hswish(a[(layer * 64 + f) * 64 + lane]));
 }
 
 /* Training backward: loss gradient dY = 2 (y - t) / (2B), back-propagated with the
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(256) void k_train_bwd(const float *__restrict__ you
     const uint16_t g1 = grp == 0 ? (uint16_t)(v >> 16) : 0;
     D4[nt][0] = (short)g0; D4[nt][1] = (short)g1; D4[nt][2] = 0; D4[nt][3] = 0;
   }
-  // dH3 = W4 dZ4^T, dZ3 = dH3 * silu'(A3)
+  // dH3 = W4 dZ4^T, dZ3 = dH3 * hswish'(A3)
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     const bf16x4_t w = load_frag(bwd, MLP_B4 + mt, lane);
@@ -240,7 +241,7 @@ __global__ __launch_bounds__(256) void k_train_bwd(const float *__restrict__ you
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int f = 16 * mt + 4 * grp + r, sl = 16 * nt + col;
-        g[r] = c[r] * silu_grad(a[(2 * 64 + f) * 64 + sl]);
+        g[r] = c[r] * hswish_grad(a[(2 * 64 + f) * 64 + sl]);
       }
       G[mt][nt] = pack4(g[0], g[1], g[2], g[3]);
 #pragma unroll
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(256) void k_train_bwd(const float *__restrict__ you
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int f = 16 * mt + 4 * grp + r, sl = 16 * nt + col;
-          g[r] = C[mt][nt][r] * silu_grad(a[(alayer * 64 + f) * 64 + sl]);
+          g[r] = C[mt][nt][r] * hswish_grad(a[(alayer * 64 + f) * 64 + sl]);
         }
         G[mt][nt] = pack4(g[0], g[1], g[2], g[3]);
 #pragma unroll
@@ -427,8 +428,10 @@ int alloc_net(maragpu_mlp **out)
     maragpu_mlp_destroy(n);
     return MARAGPU_E_NOMEM;
   }
-  HIPOK(hipMemset(n->m, 0, NPARAM * 4));
-  HIPOK(hipMemset(n->v, 0, NPARAM * 4));
+  if (hipMemset(n->m, 0, NPARAM * 4) != hipSuccess || hipMemset(n->v, 0, NPARAM * 4) != hipSuccess) {
+    maragpu_mlp_destroy(n);
+    return MARAGPU_E_HIP;
+  }
   for (int k = 0; k < MLP_IN; ++k) { n->nm.xmean[k] = 0.f; n->nm.xstd[k] = 1.f; }
   for (int k = 0; k < MLP_OUT; ++k) { n->nm.ymean[k] = 0.f; n->nm.ystd[k] = 1.f; }
   *out = n;
@@ -442,7 +445,9 @@ extern "C" {
 int maragpu_mlp_create(maragpu_mlp **out, uint64_t seed)
 {
   if (!out) return MARAGPU_E_INVALID;
-  int rc = alloc_net(out);
+  *out = nullptr;
+  maragpu_mlp *n = nullptr;
+  int rc = alloc_net(&n);
   if (rc) return rc;
   // He-style uniform initialisation from a host generator
   std::vector<float> th(NPARAM, 0.f);
@@ -464,8 +469,13 @@ int maragpu_mlp_create(maragpu_mlp **out, uint64_t seed)
   fill(OFF_W3, 64, 64);
   fill(OFF_W4, 2, 64);
   for (int i = 0; i < 2 * 64; ++i) th[OFF_W4 + i] *= 0.1f;
-  HIPOK(hipMemcpy((*out)->th, th.data(), NPARAM * 4, hipMemcpyHostToDevice));
-  return swizzle(*out, nullptr) ? MARAGPU_E_HIP : (hipDeviceSynchronize() == hipSuccess ? 0 : MARAGPU_E_HIP);
+  if (hipMemcpy(n->th, th.data(), NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      swizzle(n, nullptr) || hipDeviceSynchronize() != hipSuccess) {
+    maragpu_mlp_destroy(n);
+    return MARAGPU_E_HIP;
+  }
+  *out = n;
+  return 0;
 }
 
 int maragpu_mlp_destroy(maragpu_mlp *n)
@@ -478,15 +488,15 @@ int maragpu_mlp_destroy(maragpu_mlp *n)
 
 size_t maragpu_mlp_param_count(void) { return NPARAM; }
 
-/* Serialized form: "MLPC" magic, version 1, NPARAM, then norms (20 floats) and the
- * NPARAM master parameters (little-endian FP32). */
+/* Serialized form: "MLPC" magic, version 2 (hard-swish activations; version 1 was SiLU),
+ * NPARAM, then norms (20 floats) and the NPARAM master parameters (little-endian FP32). */
 size_t maragpu_mlp_serialized_size(void) { return 12 + 20 * 4 + NPARAM * 4; }
 
 int maragpu_mlp_save(const maragpu_mlp *n, void *buf, size_t nbytes)
 {
   if (!n || !buf || nbytes < maragpu_mlp_serialized_size()) return MARAGPU_E_INVALID;
   char *p = static_cast<char *>(buf);
-  const uint32_t hdr[3] = { 0x43504c4du, 1u, (uint32_t)NPARAM };
+  const uint32_t hdr[3] = { 0x43504c4du, kSerialVersion, (uint32_t)NPARAM };
   std::memcpy(p, hdr, 12);
   float nm[20];
   for (int k = 0; k < 6; ++k) { nm[k] = n->nm.xmean[k]; nm[6 + k] = n->nm.xstd[k]; }
@@ -503,16 +513,22 @@ int maragpu_mlp_load(maragpu_mlp **out, const void *buf, size_t nbytes)
   const char *p = static_cast<const char *>(buf);
   uint32_t hdr[3];
   std::memcpy(hdr, p, 12);
-  if (hdr[0] != 0x43504c4du || hdr[1] != 1u || hdr[2] != (uint32_t)NPARAM) return MARAGPU_E_INVALID;
-  int rc = alloc_net(out);
+  if (hdr[0] != 0x43504c4du || hdr[1] != kSerialVersion || hdr[2] != (uint32_t)NPARAM) return MARAGPU_E_INVALID;
+  *out = nullptr;
+  maragpu_mlp *n = nullptr;
+  int rc = alloc_net(&n);
   if (rc) return rc;
   float nm[20];
   std::memcpy(nm, p + 12, 80);
-  for (int k = 0; k < 6; ++k) { (*out)->nm.xmean[k] = nm[k]; (*out)->nm.xstd[k] = nm[6 + k]; }
-  for (int k = 0; k < 2; ++k) { (*out)->nm.ymean[k] = nm[12 + k]; (*out)->nm.ystd[k] = nm[14 + k]; }
-  HIPOK(hipMemcpy((*out)->th, p + 92, NPARAM * 4, hipMemcpyHostToDevice));
-  if (swizzle(*out, nullptr)) return MARAGPU_E_HIP;
-  HIPOK(hipDeviceSynchronize());
+  for (int k = 0; k < 6; ++k) { n->nm.xmean[k] = nm[k]; n->nm.xstd[k] = nm[6 + k]; }
+  for (int k = 0; k < 2; ++k) { n->nm.ymean[k] = nm[12 + k]; n->nm.ystd[k] = nm[14 + k]; }
+  // a network is handed out only once its parameters and fragments are on the device
+  if (hipMemcpy(n->th, p + 92, NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      swizzle(n, nullptr) || hipDeviceSynchronize() != hipSuccess) {
+    maragpu_mlp_destroy(n);
+    return MARAGPU_E_HIP;
+  }
+  *out = n;
   return 0;
 }
 
@@ -831,7 +847,7 @@ int maragpu_mlp_device_params(const maragpu_mlp *n, void *out_params)
   d.b2 = n->th + OFF_B2;
   d.b3 = n->th + OFF_B3;
   d.b4 = n->th + OFF_B4;
-  for (int k = 0; k < MLP_IN; ++k) { d.xmean[k] = n->nm.xmean[k]; d.xstd[k] = n->nm.xstd[k]; }
+  for (int k = 0; k < MLP_IN; ++k) { d.xmean[k] = n->nm.xmean[k]; d.xinv[k] = 1.0f / n->nm.xstd[k]; }
   for (int k = 0; k < MLP_OUT; ++k) { d.ymean[k] = n->nm.ymean[k]; d.ystd[k] = n->nm.ystd[k]; }
   std::memcpy(out_params, &d, sizeof(d));
   return 0;

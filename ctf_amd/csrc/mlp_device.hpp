@@ -1,5 +1,10 @@
-/* Learned conserved-to-primitive initial guess: a 6 -> 64 -> 64 -> 64 -> 2 MLP (SiLU)
+/* Learned conserved-to-primitive initial guess: a 6 -> 64 -> 64 -> 64 -> 2 MLP
  * evaluated with BF16 matrix cores (v_mfma_f32_16x16x16_bf16) and FP32 accumulation.
+ *
+ * Activation: hard-swish, a * clamp(a/6 + 1/2, 0, 1) (Howard et al. 2019, "Searching for
+ * MobileNetV3"), the piecewise-quadratic form of SiLU a * sigmoid(a). It costs two
+ * full-rate VALU operations where SiLU costs an exponential and a reciprocal, both
+ * quarter-rate: with 192 activations per zone that was most of the network's time.
  *
  * Layout idea. Each wave evaluates 64 samples. Every layer is computed transposed,
  *     Z^T (out x samples) = W^T (out x in) . H^T (in x samples),
@@ -51,7 +56,7 @@ struct MlpDev {
   const uint16_t *fwd;      // MLP_NFWD fragments (bf16 bits)
   const uint16_t *bwd;      // MLP_NBWD fragments
   const float *b1, *b2, *b3, *b4;  // biases (64, 64, 64, 2)
-  float xmean[MLP_IN], xstd[MLP_IN];
+  float xmean[MLP_IN], xinv[MLP_IN];   // standardisation: (x - xmean) * xinv
   float ymean[MLP_OUT], ystd[MLP_OUT];
 };
 
@@ -69,14 +74,24 @@ __device__ __forceinline__ uint32_t pack_bf16x2(float a, float b)
 __device__ __forceinline__ uint16_t f32_to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
-/* SiLU a * sigmoid(a) and its derivative, with the hardware exponential and reciprocal
- * (about 1 ulp of FP32 each): the results are rounded to bfloat16 before use. */
-__device__ __forceinline__ float sigmoid(float a) { return __builtin_amdgcn_rcpf(1.0f + __expf(-a)); }
-__device__ __forceinline__ float silu(float a) { return a * sigmoid(a); }
-__device__ __forceinline__ float silu_grad(float a)
+/* Hard-swish and its derivative (0 below -3, 1 above 3, a/3 + 1/2 between); the results
+ * are rounded to bfloat16 before use. */
+__device__ __forceinline__ float hswish(float a)
 {
-  const float s = sigmoid(a);
-  return s * (1.0f + a * (1.0f - s));
+  return a * __builtin_amdgcn_fmed3f(__builtin_fmaf(a, 1.0f / 6.0f, 0.5f), 0.0f, 1.0f);
+}
+__device__ __forceinline__ float hswish_grad(float a)
+{
+  return a <= -3.0f ? 0.0f : (a >= 3.0f ? 1.0f : __builtin_fmaf(a, 1.0f / 3.0f, 0.5f));
+}
+
+/* Orders the LDS accesses of one wave: what a lane wrote before it is visible to every
+ * lane of the wave after it (LDS operations of a wave complete in order). */
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 /* The first layer's input row of one sample: standardised features xs[MLP_IN] as
@@ -127,15 +142,16 @@ __device__ __forceinline__ void mlp_features(const double *U, double gamma, floa
   const double S2 = U[2]*U[2] + U[3]*U[3] + U[4]*U[4];
   const double B2 = U[5]*U[5] + U[6]*U[6] + U[7]*U[7];
   const double SB = U[2]*U[5] + U[3]*U[6] + U[4]*U[7];
-  // ratios in FP64 (E - |S| needs it), logarithms in FP32: the features are FP32
-  const double iD = 1.0 / D;
+  // ratios in FP64 (E - |S| needs it) with the hardware reciprocal (v_rcp_f64, far below
+  // the FP32 precision of the features), logarithms in FP32: the features are FP32
+  const double iD = __builtin_amdgcn_rcp(D);
   x[0] = logf((float)fmax(tau * iD, 1e-30));
   x[1] = log1pf((float)(S2 * iD * iD));
   x[2] = log1pf((float)(B2 * iD));
   const float q = (float)(S2 * B2);
   x[3] = q > 0.0f ? (float)SB * rsqrtf(q) : 0.0f;
   const double E = tau + D, S = sqrt(S2);
-  x[4] = logf((float)(E * E / fmax((E - S) * (E + S), 1e-300)));
+  x[4] = logf((float)(E * E * __builtin_amdgcn_rcp(fmax((E - S) * (E + S), 1e-300))));
   x[5] = (float)gamma;
 }
 
@@ -174,7 +190,7 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t *xin, const uint
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         if (act) act[(0 * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = c[r];
-        h[r] = silu(c[r]);
+        h[r] = hswish(c[r]);
       }
       H[mt][nt] = pack4(h[0], h[1], h[2], h[3]);
     }
@@ -213,7 +229,7 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t *xin, const uint
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             if (act) act[((layer + 1) * 64 + 16 * mt + 4 * grp + r) * 64 + 16 * nt + col] = C[mt][j][r];
-            h[r] = silu(C[mt][j][r]);
+            h[r] = hswish(C[mt][j][r]);
           }
           H[mt][nt] = pack4(h[0], h[1], h[2], h[3]);
         }

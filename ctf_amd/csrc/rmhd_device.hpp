@@ -41,6 +41,7 @@ template <> struct Tol<double> {
   static constexpr double newton_ftol = 1e-12;   // NewtonRaphesonSolver(15, 1e-12), riemann_hlld-rmhd.cpp:220
   static constexpr double fd_dp = 1e-8;          // HlldEquation48::Jacobian, :313
   static constexpr bool fd_relative = false;
+  static constexpr bool fd_one_sided = false;    // the reference's centred difference
   static constexpr double X_floor = 1e-8;        // :328,346
   static constexpr double small_delK = 1e-10;    // :379,404-405
   static constexpr double small_Bc = 1e-10;      // :408
@@ -55,8 +56,10 @@ template <> struct Tol<double> {
 template <> struct Tol<float> {
   static constexpr bool nested_c2p_from_average = true;
   static constexpr float newton_ftol = 2e-6f;
-  static constexpr float fd_dp = 1e-4f;          // centred difference, relative to max(|p|, 1e-3)
+  static constexpr float fd_dp = 1e-4f;          // difference step, relative to max(|p|, 1e-3)
   static constexpr bool fd_relative = true;
+  // the derivative by a one-sided difference (one more residual evaluation, not two)
+  static constexpr bool fd_one_sided = true;
   static constexpr float X_floor = 1e-8f;
   static constexpr float small_delK = 1e-6f;
   static constexpr float small_Bc = 1e-6f;
@@ -371,6 +374,9 @@ template <typename R, int DIM>
 __device__ __forceinline__ void flux_eig_pair(const R *Pl, const R *Pr, const R *Ul, const R *Ur,
                                               R *Fl, R *Fr, R &epl, R &eml, R &epr, R &emr, R gamma)
 {
+  // the four speeds are selected into place, not stored through a side-dependent address:
+  // in the rolled FP64 loop that address would put them in private memory
+  R sp0 = R(0), sm0 = R(0), sp1 = R(0), sm1 = R(0);
 #pragma unroll Shape<R>::unroll
   for (int side = 0; side < 2; ++side) {
     R P[8], U[8], F[8], ep, em;
@@ -385,9 +391,12 @@ __device__ __forceinline__ void flux_eig_pair(const R *Pl, const R *Pr, const R 
       if (side) Fr[i] = F[i];
       else Fl[i] = F[i];
     }
-    if (side) { epr = ep; emr = em; }
-    else { epl = ep; eml = em; }
+    sp1 = side ? ep : sp1;
+    sm1 = side ? em : sm1;
+    sp0 = side ? sp0 : ep;
+    sm0 = side ? sm0 : em;
   }
+  epl = sp0; eml = sm0; epr = sp1; emr = sm1;
 }
 
 /* ---------------------------------------------------------------------------
@@ -928,7 +937,9 @@ template <typename R> struct Hlld {
   }
 
   /* NewtonRaphesonSolver(15, 1e-12)::Solve with the centred-difference Jacobian of
-   * HlldEquation48::Jacobian, Mara/nrsolver.cpp:20-83, riemann_hlld-rmhd.cpp:311-319 */
+   * HlldEquation48::Jacobian, Mara/nrsolver.cpp:20-83, riemann_hlld-rmhd.cpp:311-319.
+   * FP32 (the mixed form) takes the derivative by a one-sided difference from the residual
+   * already evaluated: two evaluations per step instead of three, the same L1 error. */
   __device__ __forceinline__ bool newton(R &x, int &iters) const
   {
     using T = Tol<R>;
@@ -939,13 +950,19 @@ template <typename R> struct Hlld {
       R y = R(0), f0 = R(0), f1 = R(0);
       const R dp = T::fd_relative ? T::fd_dp * (dfabs(x) > R(1e-3) ? dfabs(x) : R(1e-3)) : T::fd_dp;
       bool ok = true;
+      // FP32: the residual, then (one-sided) f(x + dp); FP64: the reference's f(x),
+      // f(x - dp/2), f(x + dp/2)
+      constexpr int nev = T::fd_one_sided ? 2 : 3;
 #pragma unroll Shape<R>::unroll
-      for (int k = 0; k < 3; ++k) {
-        const R pk = k == 0 ? x : (k == 1 ? x - R(0.5)*dp : x + R(0.5)*dp);
+      for (int k = 0; k < nev; ++k) {
+        const R pk = k == 0 ? x : (T::fd_one_sided ? x + dp : (k == 1 ? x - R(0.5)*dp : x + R(0.5)*dp));
         R fk;
         if (!function(pk, fk)) { ok = false; break; }
-        if (k == 0) y = fk;
-        else if (k == 1) f0 = fk;
+        if (k == 0) {
+          y = fk;
+          if (T::fd_one_sided) f0 = fk;
+        }
+        else if (k == 1 && !T::fd_one_sided) f0 = fk;
         else f1 = fk;
       }
       if (!ok) { iters += it; return false; }
@@ -1100,10 +1117,11 @@ struct FluxCounters {
  * kept (to rebuild the flux side at the end and for the HLLC fallback, which averages
  * the normal field exactly as here, riemann_hllc-rmhd.cpp:60).
  * FAST = true is the common path alone: the nested c2p from the average state and the
- * first Newton attempt from EstimateSolution(0). Anything else (an invalid state, the
- * c2p cascade, the small-normal-field branch, attempts 2 and 3, the HLLC fallback)
- * returns kHlldDefer without writing f; the caller hands the interface to the full
- * solver, which then takes the same steps up to that point and gives the same result. */
+ * first Newton attempt from EstimateSolution(0), or with a small normal field the flux at
+ * EstimateSolution(1). Anything else (an invalid state, the c2p cascade, attempts 2 and
+ * 3, the HLLC fallback) returns kHlldDefer without writing f; the caller hands the
+ * interface to the full solver, which then takes the same steps up to that point and
+ * gives the same result. */
 constexpr int kHlldDefer = 100;   // FAST mode: the interface needs the full solver
 
 template <typename R, int DIM, bool FAST = false>
@@ -1204,30 +1222,33 @@ __device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f, R ga
       est0 = (R)(P[pre] + 0.5*b2);
     }
     const bool small_bn = dfabs(Pl[Bx]) < T::small_bn || dfabs(Pr[Bx]) < T::small_bn;
+    // EstimateSolution(1), eqn (55), :558-574 (the += / -= pair is kept: it rounds)
+    auto estimate1 = [&]() {
+      Fhll_tau += Fhll_ddd;
+      Uhll_tau += Uhll_ddd;
+      const R b = Uhll_tau - Fhll_Sx;
+      const R c = Fhll_tau * Uhll_Sx - Uhll_tau * Fhll_Sx;
+      const R p1 = dfabs((-b + dsqrt(b*b - R(4)*c))/R(2));
+      const R p2 = dfabs((-b - dsqrt(b*b - R(4)*c))/R(2));
+      Fhll_tau -= Fhll_ddd;
+      Uhll_tau -= Uhll_ddd;
+      return (p1 < p2) ? p1 : p2;
+    };
     if constexpr (FAST) {
-      if (small_bn) return kHlldDefer;
-      R p_star = est0;
-      success = e.newton(p_star, cnt.newton_iters) && e.flux(p_star, Pl, Pr, gamma, Fx);
+      // the small-normal-field branch (no Newton solve: the flux at EstimateSolution(1),
+      // :226-231) is as cheap as the common path; its later attempts repeat it
+      if (small_bn) success = e.flux(estimate1(), Pl, Pr, gamma, Fx);
+      else {
+        R p_star = est0;
+        success = e.newton(p_star, cnt.newton_iters) && e.flux(p_star, Pl, Pr, gamma, Fx);
+      }
       if (!success) return kHlldDefer;
     }
 #pragma unroll Shape<R>::unroll
     for (int attempt = 0; attempt < (FAST ? 0 : 3) && !success; ++attempt) {
       R p_star;
-      if (small_bn || attempt == 1) {
-        // EstimateSolution(1), eqn (55), :558-574 (the += / -= pair is kept: it rounds)
-        Fhll_tau += Fhll_ddd;
-        Uhll_tau += Uhll_ddd;
-        const R b = Uhll_tau - Fhll_Sx;
-        const R c = Fhll_tau * Uhll_Sx - Uhll_tau * Fhll_Sx;
-        const R p1 = dfabs((-b + dsqrt(b*b - R(4)*c))/R(2));
-        const R p2 = dfabs((-b - dsqrt(b*b - R(4)*c))/R(2));
-        Fhll_tau -= Fhll_ddd;
-        Uhll_tau -= Uhll_ddd;
-        p_star = (p1 < p2) ? p1 : p2;
-      }
-      else {
-        p_star = attempt == 0 ? est0 : R(0);
-      }
+      if (small_bn || attempt == 1) p_star = estimate1();
+      else p_star = attempt == 0 ? est0 : R(0);
       if (small_bn) success = e.flux(p_star, Pl, Pr, gamma, Fx);
       else success = e.newton(p_star, cnt.newton_iters) && e.flux(p_star, Pl, Pr, gamma, Fx);
     }

@@ -199,10 +199,10 @@ int maragpu_batch_eigensystem(size_t n, const double *P, double *L, double *R, d
 /* ---------------------------------------------------------------------------------
  * Learned c2p initial guess (no counterpart in the reference: the reference starts its
  * Newton solves from the cons-based estimate, Mara/rmhd-c2p.c:97-101, and from the
- * previous primitive state). A 6 -> 64 -> 64 -> 64 -> 2 SiLU network, BF16 weights and
- * FP32 accumulation on the matrix cores, maps scale-free features of U to log W and
- * log(p/D); the FP64 anton2dzw Newton solve polishes every zone from that guess and the
- * plain cascade is the fallback.
+ * previous primitive state). A 6 -> 64 -> 64 -> 64 -> 2 network (hard-swish
+ * activations), BF16 weights and FP32 accumulation on the matrix cores, maps scale-free
+ * features of U to log W and log(p/D); the FP64 anton2dzw Newton solve polishes every
+ * zone from that guess and the plain cascade is the fallback.
  * ------------------------------------------------------------------------------- */
 typedef struct maragpu_mlp maragpu_mlp;
 

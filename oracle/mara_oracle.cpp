@@ -1202,10 +1202,13 @@ void oracle_riemann_batch(long n, int solver, const double *PL, const double *PR
  * out[0..2] = mean flops per interface for x, y, z; out[3] = mean c2p flops per zone;
  * out[4..6] = mean transcendental calls per interface; out[7] = c2p transcendentals;
  * out[8..22] = per axis (add, mul, div, sqrt, transcendental) means; out[23..27] the same
- * for the c2p (out must hold 28 doubles). */
-void oracle_count_flops(void *h, const double *P_synced, long nsample, double *out)
+ * for the c2p (out must hold 28 doubles). device_path != 0 counts the device's HLLD,
+ * whose nested c2p starts from the average of the two states (PhysParams). */
+void oracle_count_flops(void *h, const double *P_synced, long nsample, double *out, int device_path)
 {
   Grid &g = *static_cast<Grid *>(h);
+  PhysParams pp_saved = g.pp;
+  g.pp.nested_c2p_from_average = device_path != 0;
   const long s0 = 8 * g.T;
   for (int dim = 1; dim <= 3; ++dim) {
     const long S = dim == 1 ? g.sx : dim == 2 ? g.sy : g.sz;
@@ -1267,6 +1270,7 @@ void oracle_count_flops(void *h, const double *P_synced, long nsample, double *o
   out[25] = (double)total.div / nsample;
   out[26] = (double)total.sqrt / nsample;
   out[27] = (double)total.trans / nsample;
+  g.pp = pp_saved;
 }
 
 }  // extern "C"

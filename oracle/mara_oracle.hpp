@@ -51,6 +51,10 @@ struct PhysParams {
   double theta = 2.0;     // weno.c plm_theta
   int IS = IS_JS96;       // weno.c IS_mode
   double sz10A = 50.0;    // weno.c shenzha10_A
+  // Flop counting only: start the HLL state's c2p with anton2dzw from the average of the
+  // two states, and run the cascade only if that fails, as the device's HLLD does (the
+  // reference runs the cascade from the cons-based estimate). Parity runs keep false.
+  bool nested_c2p_from_average = false;
 };
 
 // Accumulators the reference keeps in statics; merged by the grid driver.
@@ -1043,7 +1047,18 @@ template <class R> inline int riemann_hlld(const R *pl, const R *pr, R *f, int d
     U_hll[i] = (ap*Ur[i] - am*Ul[i] + (Fl[i] - Fr[i])) / (ap - am);
     F_hll[i] = (ap*Fl[i] - am*Fr[i] + ap*am*(Ur[i] - Ul[i])) / (ap - am);
   }
-  const int hll_c2p_failed = cons_to_prim(U_hll, P_hll, pp.gamma, pp.pfloor, &t.c2p_iterations);
+  int hll_c2p_failed = 1;
+  if (pp.nested_c2p_from_average && !c2p_check_cons(U_hll)) {
+    C2P<R> s(pp.gamma, pp.pfloor);
+    s.new_state(U_hll);
+    s.set_starting_prim(P_hll);
+    R Pg[8];
+    hll_c2p_failed = s.solve_anton2dzw(Pg);
+    t.c2p_iterations += s.iterations;
+    if (!hll_c2p_failed)
+      for (int i = 0; i < 8; ++i) P_hll[i] = Pg[i];
+  }
+  if (hll_c2p_failed) hll_c2p_failed = cons_to_prim(U_hll, P_hll, pp.gamma, pp.pfloor, &t.c2p_iterations);
 
   R U[8], F[8];
   HlldEqn<R> eqn;

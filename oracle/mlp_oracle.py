@@ -1,7 +1,8 @@
 """ORACLE — test infrastructure only.
 
-CPU evaluation (numpy) of the learned-c2p network and of the gradient of its training
-loss, with the device's precision choices restated: weights, layer inputs and the
+CPU evaluation (numpy) of the learned-c2p network (6 -> 64 -> 64 -> 64 -> 2, hard-swish
+activations) and of the gradient of its training loss, with the device's precision choices
+restated: weights, layer inputs and the
 back-propagated pre-activation gradients are rounded to bfloat16 (round to nearest
 even) wherever the device feeds them to a matrix-core product; sums are accumulated in
 float64 here (float32 on the device). The standardised input x enters the first layer
@@ -47,13 +48,14 @@ def pack(d):
     return np.concatenate([np.asarray(d[n], dtype=np.float64).ravel() for n, _ in SHAPES])
 
 
-def silu(a):
-    return a / (1.0 + np.exp(-a))
+def act(a):
+    """Hard-swish a * clamp(a/6 + 1/2, 0, 1) (Howard et al. 2019), the network's activation."""
+    return a * np.clip(a / 6.0 + 0.5, 0.0, 1.0)
 
 
-def silu_grad(a):
-    s = 1.0 / (1.0 + np.exp(-a))
-    return s * (1.0 + a * (1.0 - s))
+def act_grad(a):
+    """Its derivative: 0 below -3, 1 above 3, a/3 + 1/2 between."""
+    return np.where(a <= -3.0, 0.0, np.where(a >= 3.0, 1.0, a / 3.0 + 0.5))
 
 
 def split_input(Xs):
@@ -77,11 +79,11 @@ def forward(theta, Xs, keep=False, exact=False):
     W = {k: q(p[k]) for k in ("W1", "W2", "W3", "W4")}
     X = np.asarray(Xs, dtype=np.float64) if exact else split_input(Xs)
     A1 = X @ W["W1"].T + p["b1"]
-    H1 = q(silu(A1))
+    H1 = q(act(A1))
     A2 = H1 @ W["W2"].T + p["b2"]
-    H2 = q(silu(A2))
+    H2 = q(act(A2))
     A3 = H2 @ W["W3"].T + p["b3"]
-    H3 = q(silu(A3))
+    H3 = q(act(A3))
     Y = H3 @ W["W4"].T + p["b4"]
     if keep:
         return Y, dict(X=X, A1=A1, H1=H1, A2=A2, H2=H2, A3=A3, H3=H3, W=W)
@@ -101,13 +103,13 @@ def loss_and_grad(theta, Xs, Ts, exact=False):
     g = {}
     g["W4"] = dY.T @ c["H3"]
     g["b4"] = dY.sum(0)
-    dZ3 = q(dY @ W["W4"] * silu_grad(c["A3"]))
+    dZ3 = q(dY @ W["W4"] * act_grad(c["A3"]))
     g["W3"] = dZ3.T @ c["H2"]
     g["b3"] = dZ3.sum(0)
-    dZ2 = q(dZ3 @ W["W3"] * silu_grad(c["A2"]))
+    dZ2 = q(dZ3 @ W["W3"] * act_grad(c["A2"]))
     g["W2"] = dZ2.T @ c["H1"]
     g["b2"] = dZ2.sum(0)
-    dZ1 = q(dZ2 @ W["W2"] * silu_grad(c["A1"]))
+    dZ1 = q(dZ2 @ W["W2"] * act_grad(c["A1"]))
     g["W1"] = dZ1.T @ c["X"]
     g["b1"] = dZ1.sum(0)
     return loss, pack(g)

@@ -152,7 +152,7 @@ class Lib:
             L.oracle_set_threads.argtypes = [C.c_int]
             L.oracle_get_threads.restype = C.c_int
             L.oracle_mean_max_divB.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
-            L.oracle_count_flops.argtypes = [C.c_void_p, _dp, C.c_long, _dp]
+            L.oracle_count_flops.argtypes = [C.c_void_p, _dp, C.c_long, _dp, C.c_int]
             L.oracle_c2p_from_start_batch.argtypes = [C.c_long, _dp, _dp, _dp, C.c_double,
                                                       C.c_double, _ip, _ip]
             L.oracle_eigensystem_batch.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _ip, C.c_int,
@@ -355,10 +355,12 @@ class Solver:
         self.lib.lib.oracle_mean_max_divB(self.h, _d(P), C.byref(m), C.byref(x), C.byref(a))
         return m.value, x.value, a.value
 
-    def count_flops(self, P_synced, nsample=2000):
+    def count_flops(self, P_synced, nsample=2000, device_path=False):
+        """Mean flops per interface and per c2p zone (see oracle_count_flops); with
+        device_path the HLLD nested c2p starts from the average state, as the device's."""
         out = np.zeros(28)
         P = np.ascontiguousarray(P_synced, dtype=np.float64)
-        self.lib.lib.oracle_count_flops(self.h, _d(P), nsample, _d(out))
+        self.lib.lib.oracle_count_flops(self.h, _d(P), nsample, _d(out), int(bool(device_path)))
         return out
 
 

@@ -8,8 +8,11 @@ size:
   - every wave speed is at most the speed of light;
   - a second run from the same state gives the same bits (all reductions are fixed-order
     or order-independent).
-Plain FP64 and the mixed form (whose FP32 fluxes still enter a conservative update), and
-the other operators on the same turbulence."""
+Plain FP64, the mixed form (whose FP32 fluxes still enter a conservative update), the mixed
+form with the committed network as the c2p Newton start (c4), and the other operators on
+the same turbulence."""
+import os
+
 import numpy as np
 import pytest
 
@@ -24,8 +27,14 @@ def c2_states(gpu):
     return {N: gpu.init_turbulence(**w.resized(N).ic_kwargs()) for N in (128, 256)}
 
 
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ctf_amd", "models",
+                     "c2p_mlp.bin")
+
+
 def run(gpu, w, P0, steps):
     ctx = gpu.Context(gpu.make_config(**w.config_kwargs()))
+    if w.c2p == "learned":
+        ctx.attach_mlp(gpu.Mlp.load(MODEL))
     ctx.upload(P0)
     before = ctx.measure()["mean_cons"]
     div0 = ctx.divB()[2]
@@ -43,15 +52,18 @@ def run(gpu, w, P0, steps):
 
 # the headline (plm-split, FP64 and mixed) at its 256^3; plm-muscl at 256^3 as the bench runs
 # it; weno-split at the 128^3 of its bench record
-CASES = [("plm-split", "component", "fp64", 256), ("plm-split", "component", "mixed", 256),
-         ("plm-muscl", "component", "fp64", 256), ("weno-split", "component", "fp64", 128),
-         ("weno-split", "rmhd", "fp64", 128)]
+CASES = [("plm-split", "component", "fp64", "plain", 256),
+         ("plm-split", "component", "mixed", "plain", 256),
+         ("plm-split", "component", "mixed", "learned", 256),
+         ("plm-muscl", "component", "fp64", "plain", 256),
+         ("weno-split", "component", "fp64", "plain", 128),
+         ("weno-split", "rmhd", "fp64", "plain", 128)]
 
 
-@pytest.mark.parametrize("godunov,eigen,precision,N", CASES)
-def test_c2_full_size_invariants(gpu, c2_states, godunov, eigen, precision, N):
+@pytest.mark.parametrize("godunov,eigen,precision,c2p,N", CASES)
+def test_c2_full_size_invariants(gpu, c2_states, godunov, eigen, precision, c2p, N):
     w = workloads.replace(workloads.C2.resized(N), godunov=godunov, eigen=eigen, precision=precision,
-                          c2p="plain")
+                          c2p=c2p)
     P0 = c2_states[N]
     a = run(gpu, w, P0, 3)
     # conservation: D, tau, S, B means unchanged to round-off of the interior sums
