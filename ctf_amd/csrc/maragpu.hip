@@ -560,35 +560,38 @@ long sweep_blocks(const GridDesc &g, long BLK)
   return (g.T - 5 * S + BLK - 1) / BLK;
 }
 
-/* The calling thread's interface z of the sweep; false where it has none. */
+/* The calling thread's interface z of the sweep; false where it has none. The block
+ * arithmetic is 32-bit: it runs once per wave on the scalar unit, and 64-bit divisions
+ * there cost several hundred instructions (block counts stay far below 2^31). */
 template <int DIM>
 __device__ __forceinline__ bool sweep_interface(const GridDesc &g, long &z)
 {
   const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
-  const long BLK = blockDim.x, nb = gridDim.x, b = blockIdx.x;
-  const long q = nb >> 3;
-  const long L = b < 8 * q ? (b & 7) * q + (b >> 3) : b;
+  const unsigned BLK = blockDim.x, nb = gridDim.x, b = blockIdx.x;
+  const unsigned q = nb >> 3;
+  const unsigned L = b < 8 * q ? (b & 7) * q + (b >> 3) : b;
   if constexpr (DIM == 1) {
-    const long ntiles = (S + BLK - 1) / BLK;
-    const long nA = g.T / S - 5;
-    const long full = ntiles / kSweepChunk, per_full = nA * kSweepChunk;
-    long a, tile;
+    constexpr unsigned C = (unsigned)kSweepChunk;
+    const unsigned ntiles = (unsigned)((S + BLK - 1) / BLK);
+    const unsigned nA = (unsigned)(g.T / S - 5);
+    const unsigned full = ntiles / C, per_full = nA * C;
+    unsigned a, tile;
     if (L < full * per_full) {
-      const long rem = L % per_full;
-      a = rem / kSweepChunk;
-      tile = (L / per_full) * kSweepChunk + rem % kSweepChunk;
+      const unsigned rem = L % per_full;
+      a = rem / C;
+      tile = (L / per_full) * C + rem % C;
     }
     else {
-      const long width = ntiles - full * kSweepChunk, rem = L - full * per_full;
+      const unsigned width = ntiles - full * C, rem = L - full * per_full;
       a = rem / width;
-      tile = full * kSweepChunk + rem % width;
+      tile = full * C + rem % width;
     }
-    const long r = tile * BLK + threadIdx.x;
-    z = (2 + a) * S + r;
+    const long r = (long)tile * BLK + threadIdx.x;
+    z = (2 + (long)a) * S + r;
     return r < S;
   }
   else {
-    z = 2 * S + L * BLK + threadIdx.x;
+    z = 2 * S + (long)L * BLK + threadIdx.x;
     return z < g.T - 3 * S;
   }
 }
@@ -1220,44 +1223,6 @@ enum { C2P_CASCADE = 0, C2P_NET = 1, C2P_STORED = 2 };
 // step's first pass reused (true)
 constexpr bool kLearnedStartOncePerStep = true;
 
-/* Interior zone of the calling thread for the one-zone-per-thread update kernels, in the
- * order of the x sweep (sweep_interface): the interior's (j, k) planes cut into tiles of
- * blockDim zones, columns of kSweepChunk tiles swept over every plane i before the next
- * column, logical blocks in XCD-contiguous ranges. The fluxes a zone's CT and divergence
- * read at i - 1 are then a few blocks back, in the same L2. Returns the interior flat
- * index t (k fastest), or -1 where the thread has no zone. */
-constexpr bool kUpdateTiled = true;
-
-__device__ __forceinline__ long update_zone_index(const GridDesc &g)
-{
-  const long BLK = blockDim.x, nb = gridDim.x, b = blockIdx.x;
-  const long q = nb >> 3;
-  const long L = b < 8 * q ? (b & 7) * q + (b >> 3) : b;
-  const long NP = (long)g.N[1] * g.N[2], N0 = g.N[0];
-  const long ntiles = (NP + BLK - 1) / BLK;
-  const long full = ntiles / kSweepChunk, per_full = N0 * kSweepChunk;
-  long i, tile;
-  if (L < full * per_full) {
-    const long rem = L % per_full;
-    i = rem / kSweepChunk;
-    tile = (L / per_full) * kSweepChunk + rem % kSweepChunk;
-  }
-  else {
-    const long width = ntiles - full * kSweepChunk, rem = L - full * per_full;
-    i = rem / width;
-    tile = full * kSweepChunk + rem % width;
-  }
-  const long r = tile * BLK + threadIdx.x;
-  return r < NP ? i * NP + r : -1;
-}
-
-/* Grid of a one-zone-per-thread update kernel (with update_zone_index). */
-unsigned update_blocks(const GridDesc &g)
-{
-  const long NP = (long)g.N[1] * g.N[2];
-  return (unsigned)(g.N[0] * ((NP + kBlock - 1) / kBlock));
-}
-
 /* The stage update followed at once by the next conserved-to-primitive pass of the zone
  * (not the first pass): CT, the flux divergence and the RK combination, then the new U
  * from registers into the inversion (the plain cascade, or with LEARNED the network's
@@ -1284,8 +1249,8 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
   unsigned long long cnt[NC] = {};
   const long step = LEARNED ? (long)gridDim.x * kBlock : ni;
   for (long base = (long)blockIdx.x * kBlock; base < ni; base += step) {
-    const long t = (!LEARNED && kUpdateTiled) ? update_zone_index(g) : base + threadIdx.x;
-    const bool valid = t >= 0 && t < ni;
+    const long t = base + threadIdx.x;
+    const bool valid = t < ni;
     int i = 0, j = 0, k = 0;
     long z = 0;
     double u[8], p[8];
@@ -2351,14 +2316,13 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   auto update_c2p = [&](int kind, int pass) {
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
-    const unsigned nb = learned ? nbl : (kUpdateTiled ? update_blocks(g) : nblocks(ni));
+    const unsigned nb = learned ? nbl : nblocks(ni);
 #define UPDATE_ARGS(FT) (const FT *)c->F[0], (const FT *)c->F[1], (const FT *)c->F[2], c->U0, c->U, c->A, c->Pw, \
     g, dt, kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, net, \
     c->start
-    const unsigned nbu = kUpdateTiled ? update_blocks(g) : nblocks(ni);
     if (learned && kLearnedStartOncePerStep) {
-      if (mixed) k_update_c2p<float, C2P_STORED><<<nbu, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
-      else k_update_c2p<double, C2P_STORED><<<nbu, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
+      if (mixed) k_update_c2p<float, C2P_STORED><<<nblocks(ni), kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
+      else k_update_c2p<double, C2P_STORED><<<nblocks(ni), kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
     }
     else if (mixed && learned) k_update_c2p<float, C2P_NET><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
     else if (mixed) k_update_c2p<float, C2P_CASCADE><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
