@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -2017,6 +2018,26 @@ void launch_weno_split(maragpu_ctx *c, const PhysParams &pp)
 
 struct EventPair { int a, b; };
 
+/* Per-block partials of the measure table for maragpu_measure_noalloc, in a file-scope
+ * device array: (kMeasBlocks + 1) rows of kMeasQ doubles, 270 KB per device, held from
+ * the library's load until the process ends. Its device address on each GPU is looked up
+ * once, with that GPU current, when the first context on it is created. */
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[(kMeasBlocks + 1) * kMeasQ];
+double *g_measure_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+int measure_scratch_init(int dev)
+{
+  if (dev < 0 || dev >= kMaxDevices) return MARAGPU_E_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_measure_scratch[dev]) return 0;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MARAGPU_E_HIP;
+  g_measure_scratch[dev] = static_cast<double *>(ptr);
+  return 0;
+}
+
 }  // namespace
 
 extern "C" int maragpu_mlp_device_params(const maragpu_mlp *n, void *out_params);
@@ -2063,6 +2084,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
 
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return MARAGPU_E_HIP;
+  if (measure_scratch_init(dev)) return MARAGPU_E_HIP;
   maragpu_ctx *c = new (std::nothrow) maragpu_ctx;
   if (!c) return MARAGPU_E_NOMEM;
   c->device = dev;
@@ -2525,6 +2547,13 @@ int maragpu_measure(maragpu_ctx *c, const double *units, maragpu_measures *out)
     return MARAGPU_E_NOMEM;
   }
   return measure_into(c, units, out, c->dmeas);
+}
+
+int maragpu_measure_noalloc(maragpu_ctx *c, const double *units, maragpu_measures *out)
+{
+  if (!c || !out) return MARAGPU_E_INVALID;
+  if (c->device < 0 || c->device >= kMaxDevices || !g_measure_scratch[c->device]) return MARAGPU_E_INVALID;
+  return measure_into(c, units, out, g_measure_scratch[c->device]);
 }
 
 namespace {

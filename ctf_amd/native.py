@@ -168,6 +168,7 @@ def lib():
         "mara_set_primitive": (C.c_int, [_dp, _sz]),
         "mara_measure_mean_max_divB": (C.c_int, [_dp, _dp]),
         "maragpu_measure": (C.c_int, [_vp, _dp, C.POINTER(Measures)]),
+        "maragpu_measure_noalloc": (C.c_int, [_vp, _dp, C.POINTER(Measures)]),
         "mara_set_units": (C.c_int, [C.c_double, C.c_double, C.c_double]),
         "mara_measure_mean_velocity": (C.c_int, [_dp]),
         "mara_measure_mean_cons": (C.c_int, [_dp]),
@@ -358,6 +359,13 @@ class Context:
         m = Measures()
         u = None if units is None else (C.c_double * 3)(*[float(x) for x in units])
         check(lib().maragpu_measure(self.h, u, C.byref(m)), "measure")
+        return m.as_dict()
+
+    def measure_noalloc(self, units=None):
+        """The same table through maragpu_measure_noalloc (no device allocation)."""
+        m = Measures()
+        u = None if units is None else (C.c_double * 3)(*[float(x) for x in units])
+        check(lib().maragpu_measure_noalloc(self.h, u, C.byref(m)), "measure_noalloc")
         return m.as_dict()
 
     def set_driving(self, field: "OuField"):

@@ -143,6 +143,14 @@ typedef struct maragpu_measures {
   double max_lorentz_factor;         /*                               luaC_max_lorentz_factor (:2232-2258) */
 } maragpu_measures;
 int maragpu_measure(maragpu_ctx *ctx, const double *units, maragpu_measures *out);
+/* maragpu_measure without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). The per-block partials live in a fixed 270 KB device
+ * array of the library, one per GPU, set up when the first context on that GPU is
+ * created, so any grid size is accepted (the reduction runs over at most 1024 blocks).
+ * All calls on one GPU share that array: calls that overlap in time on one GPU (from
+ * different host threads or streams) race. Same result as maragpu_measure, bit for bit. */
+int maragpu_measure_noalloc(maragpu_ctx *ctx, const double *units, maragpu_measures *out);
 /* ---------------------------------------------------------------------------------
  * Driving and cooling: source steps applied to the committed state after every
  * successful advance, as luaC_Mara_advance does (Mara/mara.cpp:330-333).
