@@ -6,7 +6,8 @@
  *   Pc   committed primitives (the user's state)   8 x T doubles
  *   Pw   working primitives / c2p guesses          8 x T doubles
  *   U0   conserved state at the start of the step  8 x T doubles
- *   U    conserved state of the current RK stage   8 x T doubles
+ *   U    conserved state of the current RK stage   8 x T doubles (written only for a
+ *        next stage that reads it: rk2 and rk3 stages after the first)
  *   F[3] face fluxes per axis (double, or float in the mixed form)
  *   A    RK4 accumulator (rk4 only)
  *
@@ -1061,9 +1062,13 @@ __global__ __launch_bounds__(kBlock, 3) void k_ctu_c2p(const double *__restrict_
     for (int d = 0; d < 3; ++d) {
       double u[8], p[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) { u[q] = Ud[d][q * T + zi]; p[q] = P[q * T + z]; }
+      for (int q = 0; q < 8; ++q) u[q] = Ud[d][q * T + zi];
       int iters = 0, which = 0;
-      if (cons_to_prim(u, p, gamma, pfloor, true, iters, which) != 0) cnt[0] += 1;
+      if (cons_to_prim(u, p, gamma, pfloor, true, iters, which, [&](double *pp) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) pp[q] = P[q * T + z];
+          }) != 0)
+        cnt[0] += 1;
 #pragma unroll
       for (int q = 0; q < 8; ++q) Pd[d][q * T + z] = p[q];
       cnt[1] += 1;
@@ -1088,16 +1093,23 @@ enum StageKind {
   ST_RK4_4           // U <- U0 + 1/6 (A + dt L)
 };
 
+/* whether a stage of this kind reads U, the previous stage's result (the others read only
+ * U0 and A): U is written only by a stage whose successor reads it */
+constexpr bool stage_reads_u(int kind)
+{
+  return kind == ST_RK2_FINAL || kind == ST_RK3_2 || kind == ST_RK3_3;
+}
+
 template <typename FT>
 __device__ __forceinline__ double ld(const FT *__restrict__ a, long i) { return (double)a[i]; }
 
 /* One interior zone z of the stage update: CT of the three flux arrays, the flux divergence
- * and the RK combination; U (and A) written, the new U returned in unew */
+ * and the RK combination; A written, U written when write_u, the new U returned in unew */
 template <typename FT>
 __device__ __forceinline__ void update_zone(const FT *__restrict__ Fx, const FT *__restrict__ Fy,
                                             const FT *__restrict__ Fz, const double *__restrict__ U0,
                                             double *__restrict__ U, double *__restrict__ A, const GridDesc &g,
-                                            long z, double dt, int kind, double *unew)
+                                            long z, double dt, int kind, bool write_u, double *unew)
 {
 #pragma clang fp contract(off)
   const long T = g.T, sx = g.sx, sy = g.sy, sz = g.sz;
@@ -1159,7 +1171,7 @@ __device__ __forceinline__ void update_zone(const FT *__restrict__ Fx, const FT 
     case ST_RK4_3: { const double l = dt*L; A[m] = A[m] + 2.0*l; un = u0 + 1.0*l; } break;
     default: { const double l = dt*L; un = u0 + (1.0/6.0)*(A[m] + l); } break;
     }
-    U[m] = un;
+    if (write_u) U[m] = un;
     unew[q] = un;
   }
 }
@@ -1233,8 +1245,8 @@ template <typename FT, int MODE>
 __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
     const FT *__restrict__ Fx, const FT *__restrict__ Fy, const FT *__restrict__ Fz,
     const double *__restrict__ U0, double *__restrict__ U, double *__restrict__ A, double *__restrict__ P,
-    GridDesc g, double dt, int kind, double gamma, double pfloor, int ereset, double preset, double dreset,
-    DevStats *st, int pass, double *__restrict__ Uw, MlpDev net, const double2 *__restrict__ start)
+    GridDesc g, double dt, int kind, int write_u, double gamma, double pfloor, int ereset, double preset,
+    double dreset, DevStats *st, int pass, double *__restrict__ Uw, MlpDev net, const float2 *__restrict__ start)
 {
   constexpr bool LEARNED = MODE == C2P_NET;
   constexpr int NC = MODE == C2P_CASCADE ? 4 : 6;
@@ -1260,7 +1272,7 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
       j = (int)((t / g.N[2]) % g.N[1]) + g.Ng;
       i = (int)(t / ((long)g.N[2] * g.N[1])) + g.Ng;
       z = ((long)i * g.n[1] + j) * g.n[2] + k;
-      update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, z, dt, kind, u);
+      update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, z, dt, kind, write_u != 0, u);
     }
     else {
 #pragma unroll
@@ -1273,7 +1285,7 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
       if constexpr (LEARNED) learned_guess(*L, net, u, gamma, Zg, Wg);
       if (!valid) continue;
       if constexpr (MODE == C2P_STORED) {
-        const double2 zw = start[t];
+        const float2 zw = start[t];
         Zg = zw.x;
         Wg = zw.y;
       }
@@ -1286,10 +1298,11 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
     }
     else {
       if (!valid) continue;
-#pragma unroll
-      for (int q = 0; q < 8; ++q) p[q] = P[q * T + z];
       int which = 0;
-      err = cons_to_prim(u, p, gamma, pfloor, true, iters, which);
+      err = cons_to_prim(u, p, gamma, pfloor, true, iters, which, [&](double *q) {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) q[f] = P[f * T + z];
+      });
       fallback = which != 0;
     }
     finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
@@ -1297,17 +1310,18 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
   c2p_counters_out<NC>(cnt, st, pass);
 }
 
-/* The start of a step: GodunovOperator::PrimToCons of the committed state (U0 and U) and
- * the first conserved-to-primitive pass of the zone, whose previous state is the
- * zero-filled PrimitiveArray (Mara/mara.cpp:321), in one kernel; LEARNED as above. */
+/* The start of a step: GodunovOperator::PrimToCons of the committed state and the first
+ * conserved-to-primitive pass of the zone, whose previous state is the zero-filled
+ * PrimitiveArray (Mara/mara.cpp:321), in one kernel; LEARNED as above. The conserved state
+ * goes to U0 only: the first stage of every scheme reads U0 and writes U. */
 template <bool LEARNED>
 __global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const double *__restrict__ Pc,
-                                                                    double *__restrict__ U0, double *__restrict__ U,
+                                                                    double *__restrict__ U0,
                                                                     double *__restrict__ P, GridDesc g, double gamma,
                                                                     double pfloor, int ereset, double preset,
                                                                     double dreset, DevStats *st, int pass,
                                                                     double *__restrict__ Uw, MlpDev net,
-                                                                    double2 *__restrict__ start)
+                                                                    float2 *__restrict__ start)
 {
   constexpr int NC = LEARNED ? 6 : 4;
   MlpLds *L = nullptr;
@@ -1335,10 +1349,7 @@ __global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const doubl
       for (int q = 0; q < 8; ++q) p[q] = Pc[q * T + z];
       prim_to_cons(p, u, gamma - 1.0);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        U0[q * T + z] = u[q];
-        U[q * T + z] = u[q];
-      }
+      for (int q = 0; q < 8; ++q) U0[q * T + z] = u[q];
     }
     else {
 #pragma unroll
@@ -1352,7 +1363,7 @@ __global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const doubl
       double Zg, Wg;
       learned_guess(*L, net, u, gamma, Zg, Wg);
       if (!valid) continue;
-      if (start) start[t] = make_double2(Zg, Wg);
+      if (start) start[t] = make_float2((float)Zg, (float)Wg);
       int used;
       err = learned_c2p(u, p, Zg, Wg, gamma, pfloor, false, [](double *) {}, iters, used);
       fallback = !used;
@@ -1911,9 +1922,10 @@ struct maragpu_ctx {
   double flux_ms = 0.0, c2p_ms = 0.0, upd_ms = 0.0;
   int flux_n = 0, c2p_n = 0, upd_n = 0;
   // learned c2p network (not owned; see maragpu_attach_mlp) and the Newton starts of the
-  // step's first pass (one (Z, W) per interior zone), which the stage passes reuse
+  // step's first pass (one (Z, W) per interior zone, in FP32: the network computes them in
+  // FP32), which the stage passes reuse
   const maragpu_mlp *mlp = nullptr;
-  double2 *start = nullptr;
+  float2 *start = nullptr;
   // mixed form with HLLD: the FP64 redo queue of each sweep
   RedoQueue redo = { nullptr, 0u, nullptr };
   // weno-split: U with its ghost images, and the zone fluxes (8 planes) and speeds (1 plane)
@@ -2105,7 +2117,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
   if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
   if (cfg->c2p == MARAGPU_C2P_LEARNED && kLearnedStartOncePerStep &&
-      hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(double2)) != hipSuccess)
+      hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(float2)) != hipSuccess)
     return fail_free(c);
   if (cfg->godunov == MARAGPU_WENO_SPLIT) {
     const size_t rb = cfg->precision == MARAGPU_MIXED ? sizeof(float) : sizeof(double);
@@ -2335,12 +2347,12 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     faces(3, Pz, F2, sxz, 1, fl + 3);
     hipEventRecord(c->ev[p.b], c->stream);
   };
-  auto update_c2p = [&](int kind, int pass) {
+  auto update_c2p = [&](int kind, int write_u, int pass) {
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
     const unsigned nb = learned ? nbl : nblocks(ni);
 #define UPDATE_ARGS(FT) (const FT *)c->F[0], (const FT *)c->F[1], (const FT *)c->F[2], c->U0, c->U, c->A, c->Pw, \
-    g, dt, kind, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, net, \
+    g, dt, kind, write_u, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, net, \
     c->start
     if (learned && kLearnedStartOncePerStep) {
       if (mixed) k_update_c2p<float, C2P_STORED><<<nblocks(ni), kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
@@ -2360,11 +2372,11 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     EventPair p = mark(ec);
     hipEventRecord(c->ev[p.a], c->stream);
     if (learned)
-      k_p2c_c2p<true><<<nbl, kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
+      k_p2c_c2p<true><<<nbl, kBlock, 0, c->stream>>>(c->Pc, c->U0, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
                                                      c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass,
                                                      c->Uw, net, kLearnedStartOncePerStep ? c->start : nullptr);
     else
-      k_p2c_c2p<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->U, c->Pw, g, c->cfg.gamma,
+      k_p2c_c2p<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->Pw, g, c->cfg.gamma,
                                                               c->cfg.pfloor, c->cfg.ereset, c->cfg.preset,
                                                               c->cfg.dreset, c->dstats, pass, c->Uw, net, nullptr);
     if (c->Uw && c->cfg.ereset)
@@ -2377,7 +2389,7 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     if (s == 0) p2c_c2p(pass++);
     if (ctu) ctu_fluxes((int)s);
     else sweeps();
-    update_c2p(kinds[s], pass++);
+    update_c2p(kinds[s], s + 1 < kinds.size() && stage_reads_u(kinds[s + 1]) ? 1 : 0, pass++);
   }
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));

@@ -561,14 +561,25 @@ __device__ __forceinline__ int c2p_noble1dw(const C2PState &s, double Z, double 
  * means the previous state is the zero-filled PrimitiveArray (first RK stage,
  * Mara/mara.cpp:321), whose guess fails its first Newton step as NaN; those two
  * attempts are then skipped with the same outcome. `first` returns which attempt
- * succeeded (0..3) or 4. */
-template <int UNROLL = 1>
+ * succeeded (0..3) or 4.
+ * `prev(P)` loads the previous state into P where the cascade first needs it: before the
+ * first attempt from it, or on an early error (so that P ends as it would have with the
+ * state loaded up front). Callers that load P beforehand pass nothing. The first attempt
+ * starts from the estimate and nearly always succeeds, so a lazy caller reads the previous
+ * state only for the rare zone that needs it. */
+struct NoPrev {
+  __device__ void operator()(double *) const {}
+};
+template <int UNROLL = 1, typename Prev = NoPrev>
 __device__ __forceinline__ int cons_to_prim(const double *U, double *P, double gamma, double pfloor,
-                                            bool prev_valid, int &iters, int &first)
+                                            bool prev_valid, int &iters, int &first, Prev prev = Prev())
 {
   first = 4;
   int error = check_cons(U);
-  if (error) return error;
+  if (error) {
+    if (prev_valid) prev(P);
+    return error;
+  }
   C2PState s;
   c2p_new_state(s, U, gamma, pfloor);
   // attempts in the reference's order: anton2dzw from the estimate, then from the
@@ -585,7 +596,10 @@ __device__ __forceinline__ int cons_to_prim(const double *U, double *P, double g
       }
       double Z, W;
       if (guess == 0) c2p_estimate_from_cons(s, Z, W);
-      else c2p_starting_prim(s, P, Z, W);
+      else {
+        if (solver == 0) prev(P);
+        c2p_starting_prim(s, P, Z, W);
+      }
       error = solver == 0 ? c2p_anton2dzw(s, Z, W, P, iters) : c2p_noble1dw(s, Z, P, iters);
       if (!error) {
         first = 2 * solver + guess;
