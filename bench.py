@@ -319,6 +319,23 @@ def driving_record(w, steps, k1=3):
     return rec
 
 
+def weno_char_roofline(N, prec, launch_ms):
+    """The characteristic sweep against the FP64 vector peak: its hardware FP64 operations
+    per launch (SQ counters of the same launch at 256^3, profiles/weno_split_char_256.json)
+    over the live duration of one axis sweep (k_zone_flux + k_flux_weno_split_char). The
+    oracle does not count this path, so the count is the hardware's."""
+    path = os.path.join(ROOT, "profiles", "weno_split_char_256.json")
+    if N != 256 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        prof = json.load(f)
+    ops = prof["fp64" if prec == "fp64" else "mixed"]["hw_fp64_ops_per_launch"]
+    ach = ops / (launch_ms * 1e-3) / 1e12
+    return {"kernel": "k_zone_flux + k_flux_weno_split_char (one axis sweep)", "bound": "valu",
+            "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+            "ops_per_launch": ops, "avg_launch_ms": launch_ms, "ops_source": prof["what"] + ", " + prof["source"]}
+
+
 def weno_split_record(w, N, steps):
     """The weno-split operator (local Lax-Friedrichs, finite-difference WENO5, RK3) on the
     same turbulence at N^3: component-wise and RMHD-characteristic projections, FP64 and
@@ -343,6 +360,7 @@ def weno_split_record(w, N, steps):
                 stages = {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}[base.advance]
                 solves = 3 * stages * steps * (N + 1) * (N + 2 * base.Ng) ** 2
                 item["identity_projection_fraction"] = st.get("eig_identity", 0) / solves
+                item["roofline"] = weno_char_roofline(N, prec, r["flux_ms_per_launch"])
             if r is not ref:
                 item["L1_vs_component_fp64"] = l1_vs(r["P"], ref["P"], w.Ng)
             rec[f"{eigen}_{prec}"] = item
