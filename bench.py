@@ -73,26 +73,27 @@ def stage_count(w):
     return {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}[w.advance]
 
 
-def update_bytes_per_zone(w, flux_bytes, learned=False):
+def update_bytes_per_zone(w, flux_bytes, chained=False):
     """Algorithmic HBM bytes of one fused update + c2p pass per interior zone, averaged
     over the stages: the three axes' 8 fluxes and U^n read; U read by the stages that
     combine it and written by the stage before each of those (rk2, rk3); the RK4
     accumulator read and written by the stages that use it; the primitives written. The
-    previous primitives are read only where the cascade's first attempt fails (none at
-    these workloads); the learned form reads its stored FP32 start (8 B)."""
+    previous primitives are read only where the first attempt fails (none at these
+    workloads). The non-plain forms read the zone's stored FP32 start (Z, W) and write the
+    new one (8 B each)."""
     stages = stage_count(w)
     if w.advance in ("rk2", "rk3"):
         u_traffic = 2 * 64 * (stages - 1) / stages
     else:
         u_traffic = 0.0
     acc = (64 + 128 * (stages - 2) + 64) / stages if w.advance == "rk4" else 0.0
-    return 3 * 8 * flux_bytes + 64 + u_traffic + acc + 64 + (8 if learned else 0)
+    return 3 * 8 * flux_bytes + 64 + u_traffic + acc + 64 + (16 if chained else 0)
 
 
-def p2c_bytes_per_zone(learned=False):
-    """PrimToCons + the first c2p pass: P read, U^n and P written (and the learned form's
-    FP32 start)."""
-    return 192 + (8 if learned else 0)
+def p2c_bytes_per_zone(chained=False):
+    """PrimToCons + the first c2p pass: P read, U^n and P written (and, in the non-plain
+    forms, the FP32 start for the next pass)."""
+    return 192 + (8 if chained else 0)
 
 
 def form_roofline(w, r, form):
@@ -116,14 +117,15 @@ def form_roofline(w, r, form):
     if fp32:
         sweep["peak_without_packed_fp32"] = FP32_UNPACKED_TFLOPS
         sweep["frac_without_packed_fp32"] = ach / FP32_UNPACKED_TFLOPS
-    ub = update_bytes_per_zone(w, 4 if fp32 else 8, form == "learned") * nint
+    chained = form != "plain"
+    ub = update_bytes_per_zone(w, 4 if fp32 else 8, chained) * nint
     t_upd = r["update_ms_per_launch"] * 1e-3
-    upd = {"kernel": f"k_update_c2p<{'float' if fp32 else 'double'}, {'C2P_STORED' if form == 'learned' else 'C2P_CASCADE'}>",
+    upd = {"kernel": f"k_update_c2p<{'float' if fp32 else 'double'}, {'C2P_STORED' if chained else 'C2P_CASCADE'}>",
            "bound": "hbm", "achieved": ub / t_upd / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": ub / t_upd / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": ub, "avg_launch_ms": r["update_ms_per_launch"]}
-    pb = p2c_bytes_per_zone(form == "learned") * nint
+    pb = p2c_bytes_per_zone(chained) * nint
     t_p2c = r["c2p_ms_per_launch"] * 1e-3
-    p2c = {"kernel": f"k_p2c_c2p<{'true' if form == 'learned' else 'false'}>", "bound": "hbm",
+    p2c = {"kernel": f"k_p2c_c2p<{'C2P_NET' if form == 'learned' else 'C2P_CASCADE'}>", "bound": "hbm",
            "achieved": pb / t_p2c / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": pb / t_p2c / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": pb, "avg_launch_ms": r["c2p_ms_per_launch"]}
     stages = stage_count(w)

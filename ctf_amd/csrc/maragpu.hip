@@ -329,27 +329,33 @@ __device__ __forceinline__ void learned_guess(MlpLds &L, const MlpDev &net, cons
   learned_start(u[0], y0, y1, net, gamma, Zg, Wg);
 }
 
-/* The conserved-to-primitive solve of one zone in the learned form: anton2dzw from the
- * network's start; where that fails, the plain cascade (GodunovOperator::ConsToPrim,
- * Mara/rmhd.cpp:59-131) with the previous primitive state, which p holds on entry when
- * prev_valid and which is read (prev()) only then. Returns the cascade's error. */
+/* The conserved-to-primitive solve of one zone from a given Newton start (the network's,
+ * or the zone's previous solution): anton2dzw from (Zg, Wg); where that fails, the plain
+ * cascade (GodunovOperator::ConsToPrim, Mara/rmhd.cpp:59-131) with the previous primitive
+ * state, which p holds on entry when prev_valid and which is read (prev()) only then.
+ * Returns the cascade's error. With zw, the solution's (Z, W) are returned there as the
+ * zone's next start (FP32: a start needs no more). */
 template <typename Prev>
 __device__ __forceinline__ int learned_c2p(const double *u, double *p, double Zg, double Wg, double gamma,
-                                           double pfloor, bool prev_valid, Prev prev, int &iters, int &used)
+                                           double pfloor, bool prev_valid, Prev prev, int &iters, int &used,
+                                           float2 *zw = nullptr)
 {
   int which = 0;
   int err = check_cons(u);
   used = 0;
+  double zwd[2];
   if (!err) {
     C2PState s;
     c2p_new_state(s, u, gamma, pfloor);
-    err = c2p_anton2dzw(s, Zg, Wg, p, iters);
+    err = c2p_anton2dzw(s, Zg, Wg, p, iters, zwd);
     used = err == 0;
   }
   if (err) {
     prev(p);
     err = cons_to_prim(u, p, gamma, pfloor, prev_valid, iters, which);
+    if (!err && zw) c2p_zw_of_prim(p, gamma, zwd[0], zwd[1]);
   }
+  if (!err && zw) *zw = make_float2((float)zwd[0], (float)zwd[1]);
   return err;
 }
 
@@ -1229,39 +1235,33 @@ __device__ __forceinline__ void c2p_counters_out(const unsigned long long (&cnt)
 constexpr unsigned kLearnedBlocks = 4096;
 
 /* The conserved-to-primitive pass's start: the plain cascade (C2P_CASCADE), the network on
- * the zone's current U (C2P_NET), or the network's start of the step's first pass, stored
- * per zone and reused (C2P_STORED). */
+ * the zone's U (C2P_NET, the step's first pass in the learned form), or the zone's stored
+ * start, the (Z, W) of its previous solution (C2P_STORED, the stage passes of the non-plain
+ * forms). The non-plain forms (mixed precision, learned c2p) store each solution's (Z, W)
+ * as the zone's start for its next pass within the step. The step's first pass does not
+ * start from the previous step's last solution: reading it costs that pass more time than
+ * the Newton iterations it saves (0.92 against 0.88 ms at 256^3). */
 enum { C2P_CASCADE = 0, C2P_NET = 1, C2P_STORED = 2 };
-// the learned form's stage passes: the network on every pass (false) or its start of the
-// step's first pass reused (true)
-constexpr bool kLearnedStartOncePerStep = true;
 
 /* The stage update followed at once by the next conserved-to-primitive pass of the zone
  * (not the first pass): CT, the flux divergence and the RK combination, then the new U
- * from registers into the inversion (the plain cascade, or with LEARNED the network's
- * start and the cascade as fallback). With Uw (weno-split, plm-muscl) U's ghost images are
+ * from registers into the inversion (the plain cascade, or from the zone's stored start
+ * with the cascade as fallback). With Uw (weno-split, plm-muscl) U's ghost images are
  * written too. */
 template <typename FT, int MODE>
-__global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
+__global__ __launch_bounds__(kBlock, 3) void k_update_c2p(
     const FT *__restrict__ Fx, const FT *__restrict__ Fy, const FT *__restrict__ Fz,
     const double *__restrict__ U0, double *__restrict__ U, double *__restrict__ A, double *__restrict__ P,
     GridDesc g, double dt, int kind, int write_u, double gamma, double pfloor, int ereset, double preset,
-    double dreset, DevStats *st, int pass, double *__restrict__ Uw, MlpDev net, const float2 *__restrict__ start)
+    double dreset, DevStats *st, int pass, double *__restrict__ Uw, float2 *__restrict__ start)
 {
-  constexpr bool LEARNED = MODE == C2P_NET;
-  constexpr int NC = MODE == C2P_CASCADE ? 4 : 6;
-  MlpLds *L = nullptr;
-  if constexpr (LEARNED) {
-    __shared__ __align__(16) MlpLds lds;
-    L = &lds;
-    stage_net(lds, net);
-    __syncthreads();
-  }
+  static_assert(MODE == C2P_CASCADE || MODE == C2P_STORED, "stage passes start from the cascade or a stored start");
+  constexpr int NC = 4;
   const long ni = (long)g.N[0] * g.N[1] * g.N[2];
   const long T = g.T;
   unsigned long long cnt[NC] = {};
-  const long step = LEARNED ? (long)gridDim.x * kBlock : ni;
-  for (long base = (long)blockIdx.x * kBlock; base < ni; base += step) {
+  {
+    const long base = (long)blockIdx.x * kBlock;
     const long t = base + threadIdx.x;
     const bool valid = t < ni;
     int i = 0, j = 0, k = 0;
@@ -1274,48 +1274,38 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 2 : 3) void k_update_c2p(
       z = ((long)i * g.n[1] + j) * g.n[2] + k;
       update_zone<FT>(Fx, Fy, Fz, U0, U, A, g, z, dt, kind, write_u != 0, u);
     }
-    else {
+    int iters = 0, err = 0;
+    bool fallback = false;
+    if (valid) {
+      auto prev = [&](double *q) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) u[q] = q < 2 ? 1.0 : 0.0;
-    }
-    int iters = 0, err;
-    bool fallback;
-    if constexpr (MODE != C2P_CASCADE) {
-      double Zg, Wg;
-      if constexpr (LEARNED) learned_guess(*L, net, u, gamma, Zg, Wg);
-      if (!valid) continue;
+        for (int f = 0; f < 8; ++f) q[f] = P[f * T + z];
+      };
       if constexpr (MODE == C2P_STORED) {
         const float2 zw = start[t];
-        Zg = zw.x;
-        Wg = zw.y;
+        int used;
+        err = learned_c2p(u, p, (double)zw.x, (double)zw.y, gamma, pfloor, true, prev, iters, used, start + t);
+        fallback = !used;
       }
-      int used;
-      err = learned_c2p(u, p, Zg, Wg, gamma, pfloor, true, [&](double *q) {
-#pragma unroll
-        for (int f = 0; f < 8; ++f) q[f] = P[f * T + z];
-      }, iters, used);
-      fallback = !used;
+      else {
+        int which = 0;
+        err = cons_to_prim(u, p, gamma, pfloor, true, iters, which, prev);
+        fallback = which != 0;
+      }
+      finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
     }
-    else {
-      if (!valid) continue;
-      int which = 0;
-      err = cons_to_prim(u, p, gamma, pfloor, true, iters, which, [&](double *q) {
-#pragma unroll
-        for (int f = 0; f < 8; ++f) q[f] = P[f * T + z];
-      });
-      fallback = which != 0;
-    }
-    finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
   }
   c2p_counters_out<NC>(cnt, st, pass);
 }
 
 /* The start of a step: GodunovOperator::PrimToCons of the committed state and the first
  * conserved-to-primitive pass of the zone, whose previous state is the zero-filled
- * PrimitiveArray (Mara/mara.cpp:321), in one kernel; LEARNED as above. The conserved state
- * goes to U0 only: the first stage of every scheme reads U0 and writes U. */
-template <bool LEARNED>
-__global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const double *__restrict__ Pc,
+ * PrimitiveArray (Mara/mara.cpp:321), in one kernel, from the cascade or (C2P_NET) from the
+ * network's start. The conserved state goes to U0 only: the first stage of every scheme
+ * reads U0 and writes U. With start, the solution's (Z, W) are stored as the zone's start
+ * for the first stage pass. */
+template <int MODE>
+__global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 3 : 5) void k_p2c_c2p(const double *__restrict__ Pc,
                                                                     double *__restrict__ U0,
                                                                     double *__restrict__ P, GridDesc g, double gamma,
                                                                     double pfloor, int ereset, double preset,
@@ -1323,6 +1313,8 @@ __global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const doubl
                                                                     double *__restrict__ Uw, MlpDev net,
                                                                     float2 *__restrict__ start)
 {
+  static_assert(MODE == C2P_CASCADE || MODE == C2P_NET, "the first pass starts from the cascade or the network");
+  constexpr bool LEARNED = MODE == C2P_NET;
   constexpr int NC = LEARNED ? 6 : 4;
   MlpLds *L = nullptr;
   if constexpr (LEARNED) {
@@ -1363,9 +1355,8 @@ __global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const doubl
       double Zg, Wg;
       learned_guess(*L, net, u, gamma, Zg, Wg);
       if (!valid) continue;
-      if (start) start[t] = make_float2((float)Zg, (float)Wg);
       int used;
-      err = learned_c2p(u, p, Zg, Wg, gamma, pfloor, false, [](double *) {}, iters, used);
+      err = learned_c2p(u, p, Zg, Wg, gamma, pfloor, false, [](double *) {}, iters, used, start ? start + t : nullptr);
       fallback = !used;
     }
     else {
@@ -1373,6 +1364,11 @@ __global__ __launch_bounds__(kBlock, LEARNED ? 3 : 5) void k_p2c_c2p(const doubl
       int which = 0;
       err = cons_to_prim(u, p, gamma, pfloor, false, iters, which);
       fallback = which != 0;
+      if (start && !err) {
+        double Z, W;
+        c2p_zw_of_prim(p, gamma, Z, W);
+        start[t] = make_float2((float)Z, (float)W);
+      }
     }
     finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
   }
@@ -1921,9 +1917,9 @@ struct maragpu_ctx {
   std::vector<hipEvent_t> ev;
   double flux_ms = 0.0, c2p_ms = 0.0, upd_ms = 0.0;
   int flux_n = 0, c2p_n = 0, upd_n = 0;
-  // learned c2p network (not owned; see maragpu_attach_mlp) and the Newton starts of the
-  // step's first pass (one (Z, W) per interior zone, in FP32: the network computes them in
-  // FP32), which the stage passes reuse
+  // learned c2p network (not owned; see maragpu_attach_mlp) and, in the non-plain forms, each
+  // zone's Newton start for its next c2p pass within a step (the (Z, W) of its last
+  // solution, in FP32: a start needs no more)
   const maragpu_mlp *mlp = nullptr;
   float2 *start = nullptr;
   // mixed form with HLLD: the FP64 redo queue of each sweep
@@ -2116,7 +2112,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (hipMalloc(&c->dstats, sizeof(DevStats)) != hipSuccess) return fail_free(c);
   if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
   if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
-  if (cfg->c2p == MARAGPU_C2P_LEARNED && kLearnedStartOncePerStep &&
+  if ((cfg->c2p == MARAGPU_C2P_LEARNED || cfg->precision == MARAGPU_MIXED) &&
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(float2)) != hipSuccess)
     return fail_free(c);
   if (cfg->godunov == MARAGPU_WENO_SPLIT) {
@@ -2267,6 +2263,8 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   MlpDev net = {};
   if (learned && maragpu_mlp_device_params(c->mlp, &net)) return MARAGPU_E_NOMODEL;
   const unsigned nbl = nblocks(ni) < kLearnedBlocks ? nblocks(ni) : kLearnedBlocks;
+  // the non-plain forms chain each zone's Newton start from solution to solution
+  const bool chain = c->start != nullptr;
   auto sweeps = [&]() {
     for (int d = 1; d <= 3; ++d) {
       EventPair p = mark(ef);
@@ -2350,17 +2348,13 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   auto update_c2p = [&](int kind, int write_u, int pass) {
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
-    const unsigned nb = learned ? nbl : nblocks(ni);
+    const unsigned nb = nblocks(ni);
 #define UPDATE_ARGS(FT) (const FT *)c->F[0], (const FT *)c->F[1], (const FT *)c->F[2], c->U0, c->U, c->A, c->Pw, \
-    g, dt, kind, write_u, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, net, \
+    g, dt, kind, write_u, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, \
     c->start
-    if (learned && kLearnedStartOncePerStep) {
-      if (mixed) k_update_c2p<float, C2P_STORED><<<nblocks(ni), kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
-      else k_update_c2p<double, C2P_STORED><<<nblocks(ni), kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
-    }
-    else if (mixed && learned) k_update_c2p<float, C2P_NET><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
+    if (chain && mixed) k_update_c2p<float, C2P_STORED><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
+    else if (chain) k_update_c2p<double, C2P_STORED><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
     else if (mixed) k_update_c2p<float, C2P_CASCADE><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
-    else if (learned) k_update_c2p<double, C2P_NET><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
     else k_update_c2p<double, C2P_CASCADE><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
 #undef UPDATE_ARGS
     if (c->Uw && c->cfg.ereset)
@@ -2371,14 +2365,13 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   auto p2c_c2p = [&](int pass) {
     EventPair p = mark(ec);
     hipEventRecord(c->ev[p.a], c->stream);
+#define P2C_ARGS c->Pc, c->U0, c->Pw, g, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, \
+    c->dstats, pass, c->Uw, net
     if (learned)
-      k_p2c_c2p<true><<<nbl, kBlock, 0, c->stream>>>(c->Pc, c->U0, c->Pw, g, c->cfg.gamma, c->cfg.pfloor,
-                                                     c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass,
-                                                     c->Uw, net, kLearnedStartOncePerStep ? c->start : nullptr);
+      k_p2c_c2p<C2P_NET><<<nbl, kBlock, 0, c->stream>>>(P2C_ARGS, c->start);
     else
-      k_p2c_c2p<false><<<nblocks(ni), kBlock, 0, c->stream>>>(c->Pc, c->U0, c->Pw, g, c->cfg.gamma,
-                                                              c->cfg.pfloor, c->cfg.ereset, c->cfg.preset,
-                                                              c->cfg.dreset, c->dstats, pass, c->Uw, net, nullptr);
+      k_p2c_c2p<C2P_CASCADE><<<nblocks(ni), kBlock, 0, c->stream>>>(P2C_ARGS, chain ? c->start : nullptr);
+#undef P2C_ARGS
     if (c->Uw && c->cfg.ereset)
       k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
                                                                       &c->dstats->stage_reset[pass]);

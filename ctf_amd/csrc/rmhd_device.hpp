@@ -467,8 +467,10 @@ __device__ __forceinline__ int c2p_reconstruct(const C2PState &s, double Z, doub
 
 /* rmhd_c2p_solve_anton2dzw, Mara/rmhd-c2p.c:182-253 (Anton et al. 2006, eqs. 84-85).
  * The residuals and Jacobian are the reference's; the powers of 1/Z and 1/W are formed
- * from one reciprocal each per iteration (the reference divides a dozen times). */
-__device__ __forceinline__ int c2p_anton2dzw(const C2PState &s, double Z, double W, double *P, int &iters)
+ * from one reciprocal each per iteration (the reference divides a dozen times). With zw,
+ * the converged (Z, W) are returned there too (a start for the zone's next solve). */
+__device__ __forceinline__ int c2p_anton2dzw(const C2PState &s, double Z, double W, double *P, int &iters,
+                                             double *zw = nullptr)
 {
   double error = 1.0;
   int it = 0;
@@ -506,7 +508,22 @@ __device__ __forceinline__ int c2p_anton2dzw(const C2PState &s, double Z, double
     if (it == 250) { iters += it; return C2P_MAXITER; }
   }
   iters += it;
+  if (zw) {
+    zw[0] = Z;
+    zw[1] = W;
+  }
   return c2p_reconstruct(s, Z, W, P);
+}
+
+/* (Z, W) = (rho h W^2, W) of a primitive state: the Newton variables of anton2dzw at P,
+ * as c2p_starting_prim forms them (rmhd-c2p.c:108-119), with reciprocals */
+__device__ __forceinline__ void c2p_zw_of_prim(const double *P, double gamma, double &Z, double &W)
+{
+  const double V2 = P[vx]*P[vx] + P[vy]*P[vy] + P[vz]*P[vz];
+  const double W2 = rcp(1.0 - V2);
+  const double h = 1.0 + P[pre] * rcp(P[rho]) * (gamma * rcp(gamma - 1.0));
+  Z = P[rho] * h * W2;
+  W = sqrt(W2);
 }
 
 /* rmhd_c2p_solve_noble1dw, Mara/rmhd-c2p.c:261-326 (Noble et al. 2006, eq. 29) */
