@@ -42,6 +42,7 @@ template <> struct Tol<double> {
   static constexpr double fd_dp = 1e-8;          // HlldEquation48::Jacobian, :313
   static constexpr bool fd_relative = false;
   static constexpr bool fd_one_sided = false;    // the reference's centred difference
+  static constexpr double newton_accept = 0.0;   // always at least one step, as the reference
   static constexpr double X_floor = 1e-8;        // :328,346
   static constexpr double small_delK = 1e-10;    // :379,404-405
   static constexpr double small_Bc = 1e-10;      // :408
@@ -60,6 +61,9 @@ template <> struct Tol<float> {
   static constexpr bool fd_relative = true;
   // the derivative by a one-sided difference (one more residual evaluation, not two)
   static constexpr bool fd_one_sided = true;
+  // a start whose residual is below this is the root, with no step (20x below the stopping
+  // tolerance; at 3e-7 the L1 error from FP64 grows, at 1e-6 it passes 1e-6)
+  static constexpr float newton_accept = 1e-7f;
   static constexpr float X_floor = 1e-8f;
   static constexpr float small_delK = 1e-6f;
   static constexpr float small_Bc = 1e-6f;
@@ -972,7 +976,8 @@ template <typename R> struct Hlld {
   /* NewtonRaphesonSolver(15, 1e-12)::Solve with the centred-difference Jacobian of
    * HlldEquation48::Jacobian, Mara/nrsolver.cpp:20-83, riemann_hlld-rmhd.cpp:311-319.
    * FP32 (the mixed form) takes the derivative by a one-sided difference from the residual
-   * already evaluated: two evaluations per step instead of three, the same L1 error. */
+   * already evaluated: two evaluations per step instead of three, the same L1 error; and it
+   * takes a start whose residual is below newton_accept as the root without a step. */
   __device__ __forceinline__ bool newton(R &x, int &iters) const
   {
     using T = Tol<R>;
@@ -994,6 +999,9 @@ template <typename R> struct Hlld {
         if (k == 0) {
           y = fk;
           if (T::fd_one_sided) f0 = fk;
+          if constexpr (T::newton_accept > 0) {
+            if (dfabs(fk) < R(T::newton_accept)) { iters += it; return true; }
+          }
         }
         else if (k == 1 && !T::fd_one_sided) f0 = fk;
         else f1 = fk;
