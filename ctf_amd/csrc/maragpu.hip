@@ -87,6 +87,8 @@ struct RedoQueue {
   unsigned int *count;     // device counter, cleared before each sweep
   unsigned int cap;        // slots
   unsigned int *z;         // device slots: interface index z (left zone; T < 2^32)
+  uint4 *part;             // per wave of the HLLD sweep: its counters, summed by the redo
+  unsigned int part_cap;   // waves
 };
 
 struct GridDesc {
@@ -138,6 +140,69 @@ __device__ __forceinline__ void block_add(const unsigned long long (&v)[NV], uns
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; ++w) s += red[w][threadIdx.x];
     if (s) atomicAdd(dst[threadIdx.x], s);
+  }
+}
+
+/* Wave reductions by DPP moves (no LDS round trip): within each row of 16 lanes by quad
+ * permutations and row rotations, then across the four rows by the row broadcasts of
+ * lanes 15 and 31; lane 63 ends with the wave's total, which is read into a scalar. Every
+ * lane of the wave must be active. Lanes a move does not reach keep the identity (0). */
+constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E, kDppRowRor4 = 0x124, kDppRowRor8 = 0x128;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v)
+{
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, false);
+}
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = dpp_u32<CTRL, ROW_MASK>((unsigned)b), hi = dpp_u32<CTRL, ROW_MASK>((unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ unsigned wave_sum_dpp(unsigned v)
+{
+  v += dpp_u32<kDppQuadXor1>(v);
+  v += dpp_u32<kDppQuadXor2>(v);
+  v += dpp_u32<kDppRowRor4>(v);
+  v += dpp_u32<kDppRowRor8>(v);
+  v += dpp_u32<kDppRowBcast15, 0xA>(v);
+  v += dpp_u32<kDppRowBcast31, 0xC>(v);
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+/* maximum of non-negative doubles (NaN is dropped, as fmax drops it) */
+__device__ __forceinline__ double wave_max_dpp(double v)
+{
+  v = fmax(v, dpp_f64<kDppQuadXor1>(v));
+  v = fmax(v, dpp_f64<kDppQuadXor2>(v));
+  v = fmax(v, dpp_f64<kDppRowRor4>(v));
+  v = fmax(v, dpp_f64<kDppRowRor8>(v));
+  v = fmax(v, dpp_f64<kDppRowBcast15, 0xA>(v));
+  v = fmax(v, dpp_f64<kDppRowBcast31, 0xC>(v));
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+/* The sweep's closing counters, per wave: max |lambda| (the bits of a non-negative
+ * double) and the HLLD success, nested-c2p and Newton iteration counts, stored by the
+ * wave's first lane in part[wave] (x, y: the bits of max |lambda|; z: successes + 256 x
+ * nested-c2p iterations; w: Newton iterations) for the redo kernel to sum. No block
+ * barrier and no atomic: a wave retires as soon as its own interfaces are done (a
+ * per-wave atomic serialises on the counters' addresses, and the block's barrier holds
+ * finished waves until the slowest of the block is done). */
+__device__ __forceinline__ void sweep_counters_wave(double lam, bool success, unsigned c2p_iters,
+                                                    unsigned newton_iters, uint4 *part)
+{
+  const double m = wave_max_dpp(lam);
+  const unsigned ns = (unsigned)__popcll(__ballot(success));
+  const unsigned nc = wave_sum_dpp(c2p_iters), nn = wave_sum_dpp(newton_iters);
+  if ((threadIdx.x & 63) == 0) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+    part[blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)] =
+        make_uint4((unsigned)b, (unsigned)(b >> 32), ns + (nc << 8), nn);
   }
 }
 
@@ -638,9 +703,7 @@ void k_flux_hlld_fast(const double *__restrict__ P, R *__restrict__ Fout, GridDe
     lam_out = (double)lam;
   }
   queue_push(rq, defer, z);
-  block_max_lambda<kFluxBlock<R>>(lam_out, &st->lam_bits);
-  unsigned long long *const dst[3] = { &st->hlld_success, &st->hlld_c2p_iters, &st->hlld_newton_iters };
-  block_add<3, kFluxBlock<R>>(cnt, dst);
+  sweep_counters_wave(lam_out, cnt[0] != 0, (unsigned)cnt[1], (unsigned)cnt[2], rq.part);
 }
 
 /* The queued interfaces of a sweep, recomputed by the full FP64 method (reconstruction,
@@ -649,11 +712,19 @@ void k_flux_hlld_fast(const double *__restrict__ P, R *__restrict__ Fout, GridDe
 template <typename FT, int DIM, int SOLVER>
 __global__ __launch_bounds__(kBlock) void k_flux_redo(const double *__restrict__ P, FT *__restrict__ Fout,
                                                       GridDesc g, int recon, PhysParams pp, DevStats *st,
-                                                      RedoQueue rq)
+                                                      RedoQueue rq, unsigned int nparts)
 {
   const unsigned int n = min(*rq.count, rq.cap);
   double lam_out = 0.0;
   unsigned long long cnt[7] = {0, 0, 0, 0, 0, 0, 0};
+  // the sweep's per-wave counters (nparts waves; 0 for a pass without them)
+  for (unsigned int w = blockIdx.x * blockDim.x + threadIdx.x; w < nparts; w += gridDim.x * blockDim.x) {
+    const uint4 v = rq.part[w];
+    lam_out = fmax(lam_out, __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x)));
+    cnt[0] += (unsigned long long)(v.z & 255u);
+    cnt[4] += (unsigned long long)(v.z >> 8);
+    cnt[5] += (unsigned long long)v.w;
+  }
   for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const long T = g.T;
     const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
@@ -1923,7 +1994,7 @@ struct maragpu_ctx {
   const maragpu_mlp *mlp = nullptr;
   float2 *start = nullptr;
   // mixed form with HLLD: the FP64 redo queue of each sweep
-  RedoQueue redo = { nullptr, 0u, nullptr };
+  RedoQueue redo = { nullptr, 0u, nullptr, nullptr, 0u };
   // weno-split: U with its ghost images, and the zone fluxes (8 planes) and speeds (1 plane)
   // of the current sweep in the flux precision
   double *Uw = nullptr;
@@ -1986,7 +2057,7 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
   const long n = c->g.T - 5 * S;
   if (n <= 0) return;
   FT *F = static_cast<FT *>(c->F[DIM - 1]);
-  RedoQueue none = { nullptr, 0u, nullptr };
+  RedoQueue none = { nullptr, 0u, nullptr, nullptr, 0u };
   switch (c->cfg.riemann) {
   case MARAGPU_HLL:
     k_flux<R, FT, DIM, RIEMANN_HLL><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, none);
@@ -1998,10 +2069,12 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
     // the sweep, then its queued interfaces by the full FP64 solver: in the mixed form the
     // FP32 HLLD failures, in the plain form whatever left the common path
     hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
-    k_flux_hlld_fast<R, DIM><<<(unsigned)sweep_blocks<DIM>(c->g, kFluxBlock<R>), kFluxBlock<R>, 0, c->stream>>>(
-        c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, c->redo);
+    const unsigned nb = (unsigned)sweep_blocks<DIM>(c->g, kFluxBlock<R>);
+    k_flux_hlld_fast<R, DIM><<<nb, kFluxBlock<R>, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats,
+                                                                  c->redo);
     k_flux_redo<FT, DIM, RIEMANN_HLLD><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp,
-                                                                              c->dstats, c->redo);
+                                                                              c->dstats, c->redo,
+                                                                              nb * (kFluxBlock<R> / 64));
     break;
   }
 }
@@ -2131,8 +2204,18 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
     // a slot for every interface of a sweep or face pass (4 bytes a slot: half a state plane)
     const long cap = c->g.T;
     c->redo.cap = (unsigned int)cap;
+    // the HLLD sweeps' per-wave counters: one slot per wave of the largest sweep
+    long parts = 0;
+    parts = std::max(parts, sweep_blocks<1>(c->g, kFluxBlock<float>) * (kFluxBlock<float> / 64));
+    parts = std::max(parts, sweep_blocks<2>(c->g, kFluxBlock<float>) * (kFluxBlock<float> / 64));
+    parts = std::max(parts, sweep_blocks<3>(c->g, kFluxBlock<float>) * (kFluxBlock<float> / 64));
+    parts = std::max(parts, sweep_blocks<1>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
+    parts = std::max(parts, sweep_blocks<2>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
+    parts = std::max(parts, sweep_blocks<3>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
+    c->redo.part_cap = (unsigned int)parts;
     if (hipMalloc(&c->redo.count, sizeof(unsigned int)) != hipSuccess ||
-        hipMalloc(&c->redo.z, (size_t)cap * sizeof(unsigned int)) != hipSuccess)
+        hipMalloc(&c->redo.z, (size_t)cap * sizeof(unsigned int)) != hipSuccess ||
+        hipMalloc(&c->redo.part, (size_t)parts * sizeof(uint4)) != hipSuccess)
       return fail_free(c);
   }
   // fluxes outside the sweep ranges are never written; keep them defined
@@ -2160,6 +2243,7 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->start);
   hipFree(c->redo.count);
   hipFree(c->redo.z);
+  hipFree(c->redo.part);
   hipFree(c->Uw);
   hipFree(c->FZ);
   hipFree(c->dmeas);
