@@ -47,13 +47,13 @@ using namespace maragpu;
 namespace {
 
 constexpr int kBlock = 256;
-// block size of k_flux_hlld_fast: FP64 in blocks of 2 waves, so fewer waves wait at the
-// block's closing reduction (44.2-44.4 against 44.8 ms per plain 256^3 step; 1-wave blocks
-// ran at 75.9); FP32 in blocks of 4 (2-wave blocks ran its sweep at 3.8 ms against 2.0)
-template <typename R> constexpr int kFluxBlock = sizeof(R) == 4 ? 256 : 128;
+// block size of k_flux_hlld_fast (it ends without a block barrier): FP64 in one-wave blocks
+// (33.6 ms of sweeps per plain 256^3 step, against 34.1 with 2-wave and 35.5 with 4-wave
+// blocks), FP32 in blocks of 4 (8-wave blocks: 16.1 against 14.6 ms; 2-wave blocks: 14.5)
+template <typename R> constexpr int kFluxBlock = sizeof(R) == 4 ? 256 : 64;
 // occupancy targets of k_flux_hlld_fast (waves per SIMD), measured at 256^3 WENO5/HLLD:
-// FP64 at 2 (209 VGPRs, 40 B/lane scratch; 4.0 ms, against 4.7 at 3 with 184 B spilled),
-// FP32 at 5 (96 VGPRs, 44 B/lane; 2.00 ms, against 2.21 at 4 and 2.18 at 6)
+// FP64 at 2 (213 VGPRs, no scratch; 4.0 ms, against 4.7 at 3 with 184 B spilled),
+// FP32 at 5 (96 VGPRs, 40 B/lane scratch; at 4 waves, with none, 9 % slower)
 constexpr int kFastFluxWaves64 = 2;
 constexpr int kFastFluxWaves32 = 5;
 constexpr int kRedoBlocks = 1024;    // grid of k_flux_redo (a grid-stride loop over its queue)
@@ -1301,6 +1301,42 @@ __device__ __forceinline__ void c2p_counters_out(const unsigned long long (&cnt)
   }
 }
 
+/* The pass's counters, per wave (reduced with DPP moves), stored by the wave's first lane in
+ * part[wave]: x zones, y Newton iterations, z fallbacks + 65536 x failed zones (with their
+ * multiplicity), w network starts + 65536 x their fallbacks; k_c2p_parts sums the slots.
+ * The kernel then ends without a block barrier: a wave retires when its own zones are done
+ * instead of waiting for the slowest wave of its block. */
+template <int NC>
+__device__ __forceinline__ void c2p_counters_wave(const unsigned long long (&cnt)[NC], uint4 *part)
+{
+  const unsigned nz = wave_sum_dpp((unsigned)cnt[0]), ni = wave_sum_dpp((unsigned)cnt[1]);
+  const unsigned nf = wave_sum_dpp((unsigned)cnt[2]), nx = wave_sum_dpp((unsigned)cnt[3]);
+  unsigned w = 0;
+  if constexpr (NC == 6) w = wave_sum_dpp((unsigned)cnt[4]) | (wave_sum_dpp((unsigned)cnt[5]) << 16);
+  if ((threadIdx.x & 63) == 0)
+    part[blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)] = make_uint4(nz, ni, nf | (nx << 16), w);
+}
+
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_c2p_parts(const uint4 *__restrict__ part, unsigned int nw, DevStats *st,
+                                                      int pass)
+{
+  unsigned long long cnt[NC] = {};
+  for (unsigned int w = blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += gridDim.x * blockDim.x) {
+    const uint4 v = part[w];
+    cnt[0] += v.x;
+    cnt[1] += v.y;
+    cnt[2] += v.z & 0xffffu;
+    cnt[3] += v.z >> 16;
+    if constexpr (NC == 6) {
+      cnt[4] += v.w & 0xffffu;
+      cnt[5] += v.w >> 16;
+    }
+  }
+  c2p_counters_out<NC>(cnt, st, pass);
+}
+constexpr unsigned kPartBlocks = 128;   // grid of k_c2p_parts
+
 /* Grid of the learned kernels: they loop over 256-zone chunks, so the network is staged
  * into LDS once per block. */
 constexpr unsigned kLearnedBlocks = 4096;
@@ -1324,7 +1360,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_update_c2p(
     const FT *__restrict__ Fx, const FT *__restrict__ Fy, const FT *__restrict__ Fz,
     const double *__restrict__ U0, double *__restrict__ U, double *__restrict__ A, double *__restrict__ P,
     GridDesc g, double dt, int kind, int write_u, double gamma, double pfloor, int ereset, double preset,
-    double dreset, DevStats *st, int pass, double *__restrict__ Uw, float2 *__restrict__ start)
+    double dreset, DevStats *st, int pass, double *__restrict__ Uw, float2 *__restrict__ start, uint4 *part)
 {
   static_assert(MODE == C2P_CASCADE || MODE == C2P_STORED, "stage passes start from the cascade or a stored start");
   constexpr int NC = 4;
@@ -1366,7 +1402,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_update_c2p(
       finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
     }
   }
-  c2p_counters_out<NC>(cnt, st, pass);
+  c2p_counters_wave<NC>(cnt, part);
 }
 
 /* The start of a step: GodunovOperator::PrimToCons of the committed state and the first
@@ -1382,7 +1418,7 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 3 : 5) void k_p2c_c2p(con
                                                                     double pfloor, int ereset, double preset,
                                                                     double dreset, DevStats *st, int pass,
                                                                     double *__restrict__ Uw, MlpDev net,
-                                                                    float2 *__restrict__ start)
+                                                                    float2 *__restrict__ start, uint4 *part)
 {
   static_assert(MODE == C2P_CASCADE || MODE == C2P_NET, "the first pass starts from the cascade or the network");
   constexpr bool LEARNED = MODE == C2P_NET;
@@ -1443,7 +1479,7 @@ __global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 3 : 5) void k_p2c_c2p(con
     }
     finish_c2p<NC>(g, P, Uw, i, j, k, u, p, err, ereset, preset, dreset, st, pass, iters, fallback, cnt);
   }
-  c2p_counters_out<NC>(cnt, st, pass);
+  c2p_counters_wave<NC>(cnt, part);
 }
 
 /* luaC_mean_max_divB stencil, Mara/mara.cpp:2298-2314, over its loop range */
@@ -1993,6 +2029,8 @@ struct maragpu_ctx {
   // solution, in FP32: a start needs no more)
   const maragpu_mlp *mlp = nullptr;
   float2 *start = nullptr;
+  // the c2p passes' per-wave counters (one slot per wave of a pass)
+  uint4 *c2p_part = nullptr;
   // mixed form with HLLD: the FP64 redo queue of each sweep
   RedoQueue redo = { nullptr, 0u, nullptr, nullptr, 0u };
   // weno-split: U with its ghost images, and the zone fluxes (8 planes) and speeds (1 plane)
@@ -2185,6 +2223,9 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (hipMalloc(&c->dstats, sizeof(DevStats)) != hipSuccess) return fail_free(c);
   if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
   if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
+  if (hipMalloc(&c->c2p_part, (size_t)nblocks((long)cfg->N[0] * cfg->N[1] * cfg->N[2]) * (kBlock / 64) *
+                                  sizeof(uint4)) != hipSuccess)
+    return fail_free(c);
   if ((cfg->c2p == MARAGPU_C2P_LEARNED || cfg->precision == MARAGPU_MIXED) &&
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(float2)) != hipSuccess)
     return fail_free(c);
@@ -2241,6 +2282,7 @@ int maragpu_destroy(maragpu_ctx *c)
   if (c->hstats) hipHostFree(c->hstats);
   hipFree(c->ddiv);
   hipFree(c->start);
+  hipFree(c->c2p_part);
   hipFree(c->redo.count);
   hipFree(c->redo.z);
   hipFree(c->redo.part);
@@ -2435,12 +2477,13 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     const unsigned nb = nblocks(ni);
 #define UPDATE_ARGS(FT) (const FT *)c->F[0], (const FT *)c->F[1], (const FT *)c->F[2], c->U0, c->U, c->A, c->Pw, \
     g, dt, kind, write_u, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, \
-    c->start
+    c->start, c->c2p_part
     if (chain && mixed) k_update_c2p<float, C2P_STORED><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
     else if (chain) k_update_c2p<double, C2P_STORED><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
     else if (mixed) k_update_c2p<float, C2P_CASCADE><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(float));
     else k_update_c2p<double, C2P_CASCADE><<<nb, kBlock, 0, c->stream>>>(UPDATE_ARGS(double));
 #undef UPDATE_ARGS
+    k_c2p_parts<4><<<kPartBlocks, kBlock, 0, c->stream>>>(c->c2p_part, nb * (kBlock / 64), c->dstats, pass);
     if (c->Uw && c->cfg.ereset)
       k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
                                                                       &c->dstats->stage_reset[pass]);
@@ -2451,10 +2494,15 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     hipEventRecord(c->ev[p.a], c->stream);
 #define P2C_ARGS c->Pc, c->U0, c->Pw, g, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, \
     c->dstats, pass, c->Uw, net
-    if (learned)
-      k_p2c_c2p<C2P_NET><<<nbl, kBlock, 0, c->stream>>>(P2C_ARGS, c->start);
-    else
-      k_p2c_c2p<C2P_CASCADE><<<nblocks(ni), kBlock, 0, c->stream>>>(P2C_ARGS, chain ? c->start : nullptr);
+    if (learned) {
+      k_p2c_c2p<C2P_NET><<<nbl, kBlock, 0, c->stream>>>(P2C_ARGS, c->start, c->c2p_part);
+      k_c2p_parts<6><<<kPartBlocks, kBlock, 0, c->stream>>>(c->c2p_part, nbl * (kBlock / 64), c->dstats, pass);
+    }
+    else {
+      k_p2c_c2p<C2P_CASCADE><<<nblocks(ni), kBlock, 0, c->stream>>>(P2C_ARGS, chain ? c->start : nullptr, c->c2p_part);
+      k_c2p_parts<4><<<kPartBlocks, kBlock, 0, c->stream>>>(c->c2p_part, nblocks(ni) * (kBlock / 64), c->dstats,
+                                                             pass);
+    }
 #undef P2C_ARGS
     if (c->Uw && c->cfg.ereset)
       k_conserved_after_reset<<<nblocks(g.T), kBlock, 0, c->stream>>>(c->Pw, c->Uw, g, c->cfg.gamma,
