@@ -15,6 +15,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace maragpu {
 
@@ -38,6 +39,7 @@ template <> struct Tol<double> {
   // cascade's second guess) before the cascade: the same root to round-off, and the same
   // success set, since the full cascade runs whenever that attempt fails
   static constexpr bool nested_c2p_from_average = true;
+  static constexpr bool nested_c2p_fp32 = false;  // that attempt in FP64
   static constexpr double newton_ftol = 1e-12;   // NewtonRaphesonSolver(15, 1e-12), riemann_hlld-rmhd.cpp:220
   static constexpr double fd_dp = 1e-8;          // HlldEquation48::Jacobian, :313
   static constexpr bool fd_relative = false;
@@ -56,6 +58,7 @@ template <> struct Tol<double> {
 };
 template <> struct Tol<float> {
   static constexpr bool nested_c2p_from_average = true;
+  static constexpr bool nested_c2p_fp32 = true;   // that attempt in FP32 (hll_c2p_f32)
   static constexpr float newton_ftol = 2e-6f;
   static constexpr float fd_dp = 1e-4f;          // difference step, relative to max(|p|, 1e-3)
   static constexpr bool fd_relative = true;
@@ -528,6 +531,82 @@ __device__ __forceinline__ void c2p_zw_of_prim(const double *P, double gamma, do
   const double h = 1.0 + P[pre] * rcp(P[rho]) * (gamma * rcp(gamma - 1.0));
   Z = P[rho] * h * W2;
   W = sqrt(W2);
+}
+
+/* anton2dzw in FP32 for the mixed form's HLL state: the same residuals, Jacobian and
+ * bounds as c2p_anton2dzw, started from the primitive state Ps (the average of the two
+ * reconstructed states), with a step tolerance and iteration cap for single precision.
+ * Its result only seeds the FP32 p* Newton solve, whose own tolerance decides the flux.
+ * P is written only on success; a failure sends the interface to the FP64 solver. */
+constexpr float kHllC2pStep32 = 4e-6f;
+constexpr int kHllC2pIter32 = 16;
+__device__ __forceinline__ int hll_c2p_f32(const float *U, const float *Ps, float gamma, float pfloor, float *P,
+                                           int &iters)
+{
+  const float D = U[ddd], Tau = U[tau];
+  const float S2 = U[Sx]*U[Sx] + U[Sy]*U[Sy] + U[Sz]*U[Sz];
+  const float B2 = U[Bx]*U[Bx] + U[By]*U[By] + U[Bz]*U[Bz];
+  const float BS = U[Bx]*U[Sx] + U[By]*U[Sy] + U[Bz]*U[Sz];
+  const float BS2 = BS*BS;
+  const float gamf = (gamma - 1.0f) * rcp(gamma);
+  // the start from Ps (rmhd-c2p.c:108-119)
+  float Z, W;
+  {
+    const float V2 = Ps[vx]*Ps[vx] + Ps[vy]*Ps[vy] + Ps[vz]*Ps[vz];
+    const float W2 = rcp(1.0f - V2);
+    const float ir = rcp(Ps[rho]);
+    const float h = 1.0f + Ps[pre] * ir * (rcp(gamma - 1.0f) + 1.0f);
+    Z = Ps[rho] * h * W2;
+    W = dsqrt(W2);
+  }
+  const float iD = rcp(D);
+  float iZ = rcp(Z), iW = rcp(W);
+  float error = 1.0f;
+  int it = 0;
+  while (error > kHllC2pStep32) {
+    const float Z2 = Z*Z, Z3 = Z*Z2, W2 = W*W;
+    const float iZ2 = iZ*iZ, iZ3 = iZ2*iZ, iW2 = iW*iW, iW3 = iW2*iW;
+    const float Pre = (D*iW) * (Z*iW*iD - 1.0f) * gamf;
+    const float df0dZ = 2.0f*(B2+Z)*(BS2*W2 + (W2-1.0f)*Z3) * (iW2*iZ3);
+    const float df0dW = 2.0f*(B2+Z)*(B2+Z) * iW3;
+    const float df1dZ = 1.0f + BS2*iZ3 - gamf*iW2;
+    const float df1dW = B2*iW3 + (2.0f*Z - D*W)*iW3 * gamf;
+    const float f0 = -S2 + (Z+B2)*(Z+B2)*(W2-1.0f)*iW2 - (2.0f*Z+B2)*BS2*iZ2;
+    const float f1 = -Tau + Z+B2 - Pre - 0.5f*B2*iW2 - 0.5f*BS2*iZ2 - D;
+    const float idet = rcp(df0dZ*df1dW - df0dW*df1dZ);
+    const float dZ = -(df1dW*f0 - df0dW*f1) * idet;
+    const float dW = -(-df1dZ*f0 + df0dZ*f1) * idet;
+    float Z_new = Z + dZ, W_new = W + dW;
+    Z_new = (Z_new > 0.0f) ? Z_new : -Z_new;
+    Z_new = (Z_new < 1e20f) ? Z_new : Z;
+    W_new = (W_new > 1.0f) ? W_new : 1.0f;
+    W_new = (W_new < 1e12f) ? W_new : 1e12f;
+    Z = Z_new;
+    W = W_new;
+    iZ = rcp(Z);
+    iW = rcp(W);
+    error = dfabs(dZ*iZ) + dfabs(dW*iW);
+    ++it;
+    if (it == kHllC2pIter32) { iters += it; return C2P_MAXITER; }
+  }
+  iters += it;
+  // rmhd-c2p.c:121-150
+  float Q[8];
+  const float b0iW = BS * W * iZ * iW, iZB = rcp(Z + B2);
+  Q[rho] = D*iW;
+  Q[pre] = (D*iW) * (Z*iW*iD - 1.0f) * gamf;
+  Q[vx] = (U[Sx] + b0iW*U[Bx]) * iZB;
+  Q[vy] = (U[Sy] + b0iW*U[By]) * iZB;
+  Q[vz] = (U[Sz] + b0iW*U[Bz]) * iZB;
+  Q[Bx] = U[Bx];
+  Q[By] = U[By];
+  Q[Bz] = U[Bz];
+  if (Q[pre] < pfloor && pfloor > 0.0f) Q[pre] = pfloor;
+  const int err = check_prim(Q);
+  if (err) return err;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) P[i] = Q[i];
+  return C2P_SUCCESS;
 }
 
 /* rmhd_c2p_solve_noble1dw, Mara/rmhd-c2p.c:261-326 (Noble et al. 2006, eq. 29) */
@@ -1182,7 +1261,9 @@ __device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f, R ga
   Pl[Bx] = Pr[Bx] = R(0.5)*(pl[Pm::B1] + pr[Pm::B1]);
 
   Hlld<R> e;
-  double Uh[8], Ph[8];
+  // the HLL state and its primitives, in the precision of the nested c2p's first attempt
+  using RN = std::conditional_t<T::nested_c2p_fp32, float, double>;
+  RN Uh[8], Ph[8];
   R Uhll_tau, Uhll_ddd, Uhll_Sx, Fhll_tau, Fhll_ddd, Fhll_Sx;
   {
     R Ul[8], Ur[8], Fl[8], Fr[8];
@@ -1195,13 +1276,13 @@ __device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f, R ga
     const R ap = (epl > epr) ? epl : epr;
     note_lambda(lam, am, ap);
 
-    // HLL state (its primitive inversion is always FP64) and the eqn (55) inputs
+    // HLL state and the eqn (55) inputs
     const R ida = rcp(ap - am);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const R uh = (ap*Ur[i] - am*Ul[i] + (Fl[i] - Fr[i])) * ida;
-      Uh[i] = (double)uh;
-      Ph[i] = (double)(R(0.5)*(Pl[i] + Pr[i]));
+      Uh[i] = (RN)uh;
+      Ph[i] = (RN)(R(0.5)*(Pl[i] + Pr[i]));
     }
     Uhll_tau = (R)Uh[tau]; Uhll_ddd = (R)Uh[ddd]; Uhll_Sx = (R)Uh[Sx];
     Fhll_tau = (ap*Fl[tau] - am*Fr[tau] + ap*am*(Ur[tau] - Ul[tau])) * ida;
@@ -1228,7 +1309,10 @@ __device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f, R ga
     // mixed form: the HLL state lies between the two reconstructed states, so anton2dzw
     // starts from their average (the cascade's second guess) and the cascade runs only
     // if that fails; the result seeds the FP32 p* Newton solve
-    if (!check_cons(Uh)) {
+    if constexpr (sizeof(RN) == 4) {
+      if (!check_cons(Uh)) hll_c2p_failed = hll_c2p_f32(Uh, Ph, (float)gamma, (float)pfloor, Ph, cnt.c2p_iters);
+    }
+    else if (!check_cons(Uh)) {
       C2PState s;
       c2p_new_state(s, Uh, (double)gamma, pfloor);
       double Z, W;
@@ -1243,7 +1327,21 @@ __device__ __forceinline__ int riemann_hlld(const R *pl, const R *pr, R *f, R ga
   }
   if (hll_c2p_failed) {
     if constexpr (FAST) return kHlldDefer;
-    hll_c2p_failed = cons_to_prim<Shape<R>::unroll>(Uh, Ph, (double)gamma, pfloor, true, cnt.c2p_iters, c2p_first);
+    if constexpr (sizeof(RN) == 4) {
+      // the reference's cascade, in FP64, from the average state
+      double Uc[8], Pc[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        Uc[i] = (double)Uh[i];
+        Pc[i] = (double)(R(0.5)*(Pl[i] + Pr[i]));
+      }
+      hll_c2p_failed = cons_to_prim<Shape<R>::unroll>(Uc, Pc, (double)gamma, pfloor, true, cnt.c2p_iters, c2p_first);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) Ph[i] = (RN)Pc[i];
+    }
+    else {
+      hll_c2p_failed = cons_to_prim<Shape<R>::unroll>(Uh, Ph, (double)gamma, pfloor, true, cnt.c2p_iters, c2p_first);
+    }
   }
 
   bool success = false;
