@@ -187,22 +187,23 @@ __device__ __forceinline__ double wave_max_dpp(double v)
 }
 
 /* The sweep's closing counters, per wave: max |lambda| (the bits of a non-negative
- * double) and the HLLD success, nested-c2p and Newton iteration counts, stored by the
- * wave's first lane in part[wave] (x, y: the bits of max |lambda|; z: successes + 256 x
- * nested-c2p iterations; w: Newton iterations) for the redo kernel to sum. No block
+ * double) and the HLLD success, nested-c2p and Newton iteration counts and failed states,
+ * stored by the wave's first lane in part[wave] (x, y: the bits of max |lambda|; z:
+ * successes + 256 x nested-c2p iterations; w: Newton iterations + 65536 x failed states)
+ * for the redo kernel to sum. No block
  * barrier and no atomic: a wave retires as soon as its own interfaces are done (a
  * per-wave atomic serialises on the counters' addresses, and the block's barrier holds
  * finished waves until the slowest of the block is done). */
 __device__ __forceinline__ void sweep_counters_wave(double lam, bool success, unsigned c2p_iters,
-                                                    unsigned newton_iters, uint4 *part)
+                                                    unsigned newton_iters, bool failed, uint4 *part)
 {
   const double m = wave_max_dpp(lam);
-  const unsigned ns = (unsigned)__popcll(__ballot(success));
+  const unsigned ns = (unsigned)__popcll(__ballot(success)), nf = (unsigned)__popcll(__ballot(failed));
   const unsigned nc = wave_sum_dpp(c2p_iters), nn = wave_sum_dpp(newton_iters);
   if ((threadIdx.x & 63) == 0) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(m);
     part[blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)] =
-        make_uint4((unsigned)b, (unsigned)(b >> 32), ns + (nc << 8), nn);
+        make_uint4((unsigned)b, (unsigned)(b >> 32), ns + (nc << 8), nn + (nf << 16));
   }
 }
 
@@ -703,7 +704,7 @@ void k_flux_hlld_fast(const double *__restrict__ P, R *__restrict__ Fout, GridDe
     lam_out = (double)lam;
   }
   queue_push(rq, defer, z);
-  sweep_counters_wave(lam_out, cnt[0] != 0, (unsigned)cnt[1], (unsigned)cnt[2], rq.part);
+  sweep_counters_wave(lam_out, cnt[0] != 0, (unsigned)cnt[1], (unsigned)cnt[2], false, rq.part);
 }
 
 /* The queued interfaces of a sweep, recomputed by the full FP64 method (reconstruction,
@@ -723,7 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_flux_redo(const double *__restrict__
     lam_out = fmax(lam_out, __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x)));
     cnt[0] += (unsigned long long)(v.z & 255u);
     cnt[4] += (unsigned long long)(v.z >> 8);
-    cnt[5] += (unsigned long long)v.w;
+    cnt[5] += (unsigned long long)(v.w & 0xffffu);
   }
   for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const long T = g.T;
@@ -1012,9 +1013,7 @@ void k_ctu_faces_fast(const double *__restrict__ P, const double *__restrict__ P
     }
   }
   queue_push(rq, defer, z);
-  block_max_lambda(lam_out, &st->lam_bits);
-  unsigned long long *const dst[4] = { fail, &st->hlld_success, &st->hlld_c2p_iters, &st->hlld_newton_iters };
-  block_add<4>(cnt, dst);
+  sweep_counters_wave(lam_out, cnt[1] != 0, (unsigned)cnt[2], (unsigned)cnt[3], cnt[0] != 0, rq.part);
 }
 
 /* The queued faces of k_ctu_faces_fast by the full HLLD method (attempt ladder, HLLC
@@ -1023,13 +1022,23 @@ void k_ctu_faces_fast(const double *__restrict__ P, const double *__restrict__ P
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_ctu_faces_redo(const double *__restrict__ P, const double *__restrict__ Ps,
                                                            double *__restrict__ F, GridDesc g, PhysParams pp, int keep,
-                                                           DevStats *st, RedoQueue rq)
+                                                           DevStats *st, unsigned long long *fail, RedoQueue rq,
+                                                           unsigned int nparts)
 {
   const unsigned int n = min(*rq.count, rq.cap);
   const long T = g.T, sxz = g.sx;
   const long S = DIM == 1 ? g.sx : DIM == 2 ? g.sy : g.sz;
   double lam_out = 0.0;
-  unsigned long long cnt[5] = {0, 0, 0, 0, 0};
+  unsigned long long cnt[6] = {0, 0, 0, 0, 0, 0};
+  // the fast pass's per-wave counters
+  for (unsigned int w = blockIdx.x * blockDim.x + threadIdx.x; w < nparts; w += gridDim.x * blockDim.x) {
+    const uint4 v = rq.part[w];
+    lam_out = fmax(lam_out, __longlong_as_double((long long)(((unsigned long long)v.y << 32) | v.x)));
+    cnt[0] += (unsigned long long)(v.z & 255u);
+    cnt[2] += (unsigned long long)(v.z >> 8);
+    cnt[3] += (unsigned long long)(v.w & 0xffffu);
+    cnt[5] += (unsigned long long)(v.w >> 16);
+  }
   for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     const long z = (long)rq.z[k];
     double pl[8], pr[8], f[8], lam = 0.0;
@@ -1048,9 +1057,9 @@ __global__ __launch_bounds__(kBlock) void k_ctu_faces_redo(const double *__restr
     cnt[4] += 1;
   }
   block_max_lambda(lam_out, &st->lam_bits);
-  unsigned long long *const dst[5] = { &st->hlld_success, &st->hlld_failure, &st->hlld_c2p_iters,
-                                       &st->hlld_newton_iters, &st->redo };
-  block_add<5>(cnt, dst);
+  unsigned long long *const dst[6] = { &st->hlld_success, &st->hlld_failure, &st->hlld_c2p_iters,
+                                       &st->hlld_newton_iters, &st->redo, fail };
+  block_add<6>(cnt, dst);
 }
 
 /* the physical flux along DIM of the state P (U from it) */
@@ -2253,6 +2262,7 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
     parts = std::max(parts, sweep_blocks<1>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
     parts = std::max(parts, sweep_blocks<2>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
     parts = std::max(parts, sweep_blocks<3>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
+    parts = std::max(parts, (long)nblocks(c->g.T) * (kBlock / 64));   // plm-muscl's face passes
     c->redo.part_cap = (unsigned int)parts;
     if (hipMalloc(&c->redo.count, sizeof(unsigned int)) != hipSuccess ||
         hipMalloc(&c->redo.z, (size_t)cap * sizeof(unsigned int)) != hipSuccess ||
@@ -2450,9 +2460,10 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
         if (dim == 1) k_ctu_faces_fast<1><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
         if (dim == 2) k_ctu_faces_fast<2><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
         if (dim == 3) k_ctu_faces_fast<3><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
-        if (dim == 1) k_ctu_faces_redo<1><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, c->redo);
-        if (dim == 2) k_ctu_faces_redo<2><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, c->redo);
-        if (dim == 3) k_ctu_faces_redo<3><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, c->redo);
+        const unsigned np = nblocks(n) * (kBlock / 64);
+        if (dim == 1) k_ctu_faces_redo<1><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo, np);
+        if (dim == 2) k_ctu_faces_redo<2><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo, np);
+        if (dim == 3) k_ctu_faces_redo<3><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo, np);
       }
       else {
         if (dim == 1) k_ctu_faces<1><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, rs, begin, keep, c->dstats, f);
