@@ -2022,6 +2022,11 @@ struct maragpu_ctx {
   GridDesc g;
   int device = 0;                    // the HIP device current at create
   hipStream_t stream = nullptr;
+  // HLLD sweeps: the redo of an axis runs on `side`, beside the next axis's sweep, and the
+  // update waits for all three (redo_done); each axis has its own queue and counter slots
+  hipStream_t side = nullptr;
+  hipEvent_t sweep_done[3] = {nullptr, nullptr, nullptr}, redo_done[3] = {nullptr, nullptr, nullptr};
+  unsigned redo_pending = 0;         // bit d: the redo of axis d+1 may still run on `side`
   double *Pc = nullptr, *Pw = nullptr, *U0 = nullptr, *U = nullptr, *A = nullptr;
   void *F[3] = {nullptr, nullptr, nullptr};
   DevStats *dstats = nullptr;
@@ -2041,7 +2046,7 @@ struct maragpu_ctx {
   // the c2p passes' per-wave counters (one slot per wave of a pass)
   uint4 *c2p_part = nullptr;
   // mixed form with HLLD: the FP64 redo queue of each sweep
-  RedoQueue redo = { nullptr, 0u, nullptr, nullptr, 0u };
+  RedoQueue redo[3] = {};           // one per axis (plm-muscl's face passes use the first)
   // weno-split: U with its ghost images, and the zone fluxes (8 planes) and speeds (1 plane)
   // of the current sweep in the flux precision
   double *Uw = nullptr;
@@ -2115,13 +2120,18 @@ void launch_flux(maragpu_ctx *c, const PhysParams &pp)
   default:
     // the sweep, then its queued interfaces by the full FP64 solver: in the mixed form the
     // FP32 HLLD failures, in the plain form whatever left the common path
-    hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
+    // the redo of this axis runs on the side stream while the next axis is swept: it reads
+    // only P and this axis's queue and slots, and writes only this axis's fluxes
+    RedoQueue &q = c->redo[DIM - 1];
+    hipMemsetAsync(q.count, 0, sizeof(unsigned int), c->stream);
     const unsigned nb = (unsigned)sweep_blocks<DIM>(c->g, kFluxBlock<R>);
-    k_flux_hlld_fast<R, DIM><<<nb, kFluxBlock<R>, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats,
-                                                                  c->redo);
-    k_flux_redo<FT, DIM, RIEMANN_HLLD><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp,
-                                                                              c->dstats, c->redo,
-                                                                              nb * (kFluxBlock<R> / 64));
+    k_flux_hlld_fast<R, DIM><<<nb, kFluxBlock<R>, 0, c->stream>>>(c->Pw, F, c->g, c->cfg.recon, pp, c->dstats, q);
+    hipEventRecord(c->sweep_done[DIM - 1], c->stream);
+    hipStreamWaitEvent(c->side, c->sweep_done[DIM - 1], 0);
+    k_flux_redo<FT, DIM, RIEMANN_HLLD><<<kRedoBlocks, kBlock, 0, c->side>>>(c->Pw, F, c->g, c->cfg.recon, pp,
+                                                                            c->dstats, q, nb * (kFluxBlock<R> / 64));
+    hipEventRecord(c->redo_done[DIM - 1], c->side);
+    c->redo_pending |= 1u << (DIM - 1);
     break;
   }
 }
@@ -2220,6 +2230,11 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   c->g = make_grid(*cfg);
   if (c->g.T >= (1L << 32)) { delete c; return MARAGPU_E_INVALID; }   // 32-bit queue slots
   if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return MARAGPU_E_HIP; }
+  if (hipStreamCreate(&c->side) != hipSuccess) return fail_free(c);
+  for (int d = 0; d < 3; ++d)
+    if (hipEventCreateWithFlags(&c->sweep_done[d], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->redo_done[d], hipEventDisableTiming) != hipSuccess)
+      return fail_free(c);
   const size_t nd = 8 * (size_t)c->g.T;
   const size_t fbytes = nd * (cfg->precision == MARAGPU_MIXED ? sizeof(float) : sizeof(double));
   if (hipMalloc(&c->Pc, nd * sizeof(double)) != hipSuccess) return fail_free(c);
@@ -2253,7 +2268,6 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (cfg->riemann == MARAGPU_HLLD && cfg->godunov != MARAGPU_WENO_SPLIT) {
     // a slot for every interface of a sweep or face pass (4 bytes a slot: half a state plane)
     const long cap = c->g.T;
-    c->redo.cap = (unsigned int)cap;
     // the HLLD sweeps' per-wave counters: one slot per wave of the largest sweep
     long parts = 0;
     parts = std::max(parts, sweep_blocks<1>(c->g, kFluxBlock<float>) * (kFluxBlock<float> / 64));
@@ -2263,11 +2277,14 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
     parts = std::max(parts, sweep_blocks<2>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
     parts = std::max(parts, sweep_blocks<3>(c->g, kFluxBlock<double>) * (kFluxBlock<double> / 64));
     parts = std::max(parts, (long)nblocks(c->g.T) * (kBlock / 64));   // plm-muscl's face passes
-    c->redo.part_cap = (unsigned int)parts;
-    if (hipMalloc(&c->redo.count, sizeof(unsigned int)) != hipSuccess ||
-        hipMalloc(&c->redo.z, (size_t)cap * sizeof(unsigned int)) != hipSuccess ||
-        hipMalloc(&c->redo.part, (size_t)parts * sizeof(uint4)) != hipSuccess)
-      return fail_free(c);
+    for (RedoQueue &q : c->redo) {
+      q.cap = (unsigned int)cap;
+      q.part_cap = (unsigned int)parts;
+      if (hipMalloc(&q.count, sizeof(unsigned int)) != hipSuccess ||
+          hipMalloc(&q.z, (size_t)cap * sizeof(unsigned int)) != hipSuccess ||
+          hipMalloc(&q.part, (size_t)parts * sizeof(uint4)) != hipSuccess)
+        return fail_free(c);
+    }
   }
   // fluxes outside the sweep ranges are never written; keep them defined
   for (int d = 0; d < 3; ++d)
@@ -2286,6 +2303,7 @@ int maragpu_destroy(maragpu_ctx *c)
 {
   if (!c) return MARAGPU_E_INVALID;
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->side) hipStreamSynchronize(c->side);
   hipFree(c->Pc); hipFree(c->Pw); hipFree(c->U0); hipFree(c->U); hipFree(c->A);
   for (int d = 0; d < 3; ++d) hipFree(c->F[d]);
   hipFree(c->dstats);
@@ -2293,9 +2311,11 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->ddiv);
   hipFree(c->start);
   hipFree(c->c2p_part);
-  hipFree(c->redo.count);
-  hipFree(c->redo.z);
-  hipFree(c->redo.part);
+  for (RedoQueue &q : c->redo) {
+    hipFree(q.count);
+    hipFree(q.z);
+    hipFree(q.part);
+  }
   hipFree(c->Uw);
   hipFree(c->FZ);
   hipFree(c->dmeas);
@@ -2304,6 +2324,11 @@ int maragpu_destroy(maragpu_ctx *c)
   hipFree(c->Fdrive);
   hipFree(c->dcool);
   for (auto &e : c->ev) if (e) hipEventDestroy(e);
+  for (int d = 0; d < 3; ++d) {
+    if (c->sweep_done[d]) hipEventDestroy(c->sweep_done[d]);
+    if (c->redo_done[d]) hipEventDestroy(c->redo_done[d]);
+  }
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return MARAGPU_OK;
@@ -2401,8 +2426,10 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
   const unsigned nbl = nblocks(ni) < kLearnedBlocks ? nblocks(ni) : kLearnedBlocks;
   // the non-plain forms chain each zone's Newton start from solution to solution
   const bool chain = c->start != nullptr;
+  // the three axis sweeps, z first: their results do not depend on the order, and with HLLD
+  // the redo of the last axis cannot overlap a sweep, so it is the cheapest one (x at c2)
   auto sweeps = [&]() {
-    for (int d = 1; d <= 3; ++d) {
+    for (int d = 3; d >= 1; --d) {
       EventPair p = mark(ef);
       hipEventRecord(c->ev[p.a], c->stream);
       if (c->cfg.godunov == MARAGPU_WENO_SPLIT) {
@@ -2455,15 +2482,15 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     auto faces = [&](int dim, const double *Ps, double *F, long begin, int keep, unsigned long long *f) {
       const long n = T - sxz - begin;
       if (n <= 0) return;
-      if (c->redo.z) {
-        hipMemsetAsync(c->redo.count, 0, sizeof(unsigned int), c->stream);
-        if (dim == 1) k_ctu_faces_fast<1><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
-        if (dim == 2) k_ctu_faces_fast<2><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
-        if (dim == 3) k_ctu_faces_fast<3><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo);
+      if (c->redo[0].z) {
+        hipMemsetAsync(c->redo[0].count, 0, sizeof(unsigned int), c->stream);
+        if (dim == 1) k_ctu_faces_fast<1><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo[0]);
+        if (dim == 2) k_ctu_faces_fast<2><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo[0]);
+        if (dim == 3) k_ctu_faces_fast<3><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, begin, c->dstats, f, c->redo[0]);
         const unsigned np = nblocks(n) * (kBlock / 64);
-        if (dim == 1) k_ctu_faces_redo<1><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo, np);
-        if (dim == 2) k_ctu_faces_redo<2><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo, np);
-        if (dim == 3) k_ctu_faces_redo<3><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo, np);
+        if (dim == 1) k_ctu_faces_redo<1><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo[0], np);
+        if (dim == 2) k_ctu_faces_redo<2><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo[0], np);
+        if (dim == 3) k_ctu_faces_redo<3><<<kRedoBlocks, kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, keep, c->dstats, f, c->redo[0], np);
       }
       else {
         if (dim == 1) k_ctu_faces<1><<<nblocks(n), kBlock, 0, c->stream>>>(c->Pw, Ps, F, g, pp, rs, begin, keep, c->dstats, f);
@@ -2482,7 +2509,14 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     faces(3, Pz, F2, sxz, 1, fl + 3);
     hipEventRecord(c->ev[p.b], c->stream);
   };
+  // the update reads the fluxes of all three axes: wait for the redo kernels on the side stream
+  auto join_redo = [&]() {
+    for (int d = 0; d < 3; ++d)
+      if (c->redo_pending & (1u << d)) hipStreamWaitEvent(c->stream, c->redo_done[d], 0);
+    c->redo_pending = 0;
+  };
   auto update_c2p = [&](int kind, int write_u, int pass) {
+    join_redo();
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
     const unsigned nb = nblocks(ni);
@@ -2527,6 +2561,7 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     else sweeps();
     update_c2p(kinds[s], s + 1 < kinds.size() && stage_reads_u(kinds[s + 1]) ? 1 : 0, pass++);
   }
+  join_redo();
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipEventRecord(c->ev[evi], c->stream));
   const int ev_end = evi++;
