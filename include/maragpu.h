@@ -1,6 +1,8 @@
 /* maragpu — MI355X (gfx950) relativistic-MHD update, C ABI.
  *
- * One context owns one device-resident grid and one HIP stream. Nothing in this header
+ * One context owns one device-resident grid and two HIP streams (the second runs the HLLD
+ * sweeps' redo kernels beside the next sweep; maragpu_advance returns with both drained).
+ * Nothing in this header
  * uses HIP or torch types; all pointers are host pointers unless stated otherwise.
  * Every entry point returns instead of throwing. Return codes: 0 success, > 0 a count
  * (failed zones, for maragpu_advance), < 0 an error (maragpu_strerror names it).
