@@ -2,8 +2,8 @@
  *
  * One context owns one device-resident grid and two HIP streams (the second runs the HLLD
  * sweeps' redo kernels beside the next sweep; maragpu_advance returns with both drained).
- * Nothing in this header
- * uses HIP or torch types; all pointers are host pointers unless stated otherwise.
+ * Nothing in this header uses HIP or torch types; all pointers are host pointers unless
+ * stated otherwise.
  * Every entry point returns instead of throwing. Return codes: 0 success, > 0 a count
  * (failed zones, for maragpu_advance), < 0 an error (maragpu_strerror names it).
  *
