@@ -1420,8 +1420,10 @@ __global__ __launch_bounds__(kBlock, 3) void k_update_c2p(
  * network's start. The conserved state goes to U0 only: the first stage of every scheme
  * reads U0 and writes U. With start, the solution's (Z, W) are stored as the zone's start
  * for the first stage pass. */
+// occupancy: 4 waves per SIMD for the cascade (8 % faster than 5, where it keeps 36 B per
+// lane in scratch), 3 with the network (2: the same; 4: 1.9x slower)
 template <int MODE>
-__global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 3 : 5) void k_p2c_c2p(const double *__restrict__ Pc,
+__global__ __launch_bounds__(kBlock, MODE == C2P_NET ? 3 : 4) void k_p2c_c2p(const double *__restrict__ Pc,
                                                                     double *__restrict__ U0,
                                                                     double *__restrict__ P, GridDesc g, double gamma,
                                                                     double pfloor, int ereset, double preset,
