@@ -110,8 +110,10 @@ __device__ __forceinline__ float dsin(float x) { return sinf(x); }
 
 /* Reciprocals. A correctly rounded FP64 division costs about ten instructions on
  * gfx950 (div_scale x2, rcp, five fma, div_fmas, div_fixup); the hardware reciprocal
- * refined by two Newton steps costs five and is within an ulp. Where the refinement is
- * not finite (x = 0, inf, NaN) the raw reciprocal is returned, so 1/0 stays inf. */
+ * refined by two Newton steps costs five and is within an ulp. The refinement is not
+ * finite where x is 0 or inf (0 x inf); v_div_fixup_f64 then gives the IEEE quotient
+ * (1/±0 = ±inf, 1/±inf = ±0, NaN for NaN) and leaves every other result unchanged, in one
+ * instruction where a NaN test and select take three (4 % of the FP64 sweep). */
 __device__ __forceinline__ double rcp(double x)
 {
   const double y = __builtin_amdgcn_rcp(x);
@@ -119,7 +121,7 @@ __device__ __forceinline__ double rcp(double x)
   double r = fma(y, e, y);
   e = fma(-x, r, 1.0);
   r = fma(r, e, r);
-  return r == r ? r : y;
+  return __builtin_amdgcn_div_fixup(r, x, 1.0);
 }
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
