@@ -274,10 +274,16 @@ template <typename R> __device__ __forceinline__ void quartic_extremes(R d4, R d
  * All four roots are real and Q has a positive leading coefficient, so Newton's method
  * started outside the roots moves monotonically onto the extreme one. Evaluated in this
  * form, Q keeps its relative accuracy near the roots even when they crowd within 1/W^2 of
- * the flow speed, which is where single-precision Ferrari loses every digit. */
+ * the flow speed, which is where single-precision Ferrari loses every digit.
+ * Stopping: a step below 1e-7, or a next step predicted below 1e-7. Once convergence is
+ * quadratic, s_{n+1} ~ M s_n^2 with M ~ s_n / s_{n-1}^2, so the step after s_n is about
+ * s_n^3 / s_{n-1}^2; near a double root, where convergence is linear (s_{n+1} ~ s_n / 2),
+ * the estimate is s_n / 4, so there the loop ends at a step of 4e-7 instead of 1e-7. Every
+ * iterate stays outside the root, so an early stop can only overstate the speed. */
 template <typename R>
 __device__ __forceinline__ R quartic_root_from_outside(R K, R L, R b0, R bi, R v, R x)
 {
+  R sp = R(0);   // the previous |step|: no prediction from the first step
 #pragma unroll 1
   for (int it = 0; it < 24; ++it) {
     const R y = x - v, y2 = y*y, c = b0*x - bi;
@@ -289,7 +295,9 @@ __device__ __forceinline__ R quartic_root_from_outside(R K, R L, R b0, R bi, R v
     const R xn = x - step;
     if (!(dfabs(xn) <= R(1))) break;      // not finite, or out of the light cone: keep x
     x = xn;
-    if (!(dfabs(step) > R(1e-7))) break;
+    const R s = dfabs(step);
+    if (!(s > R(1e-7)) || s*s*s < R(1e-7)*sp*sp) break;
+    sp = s;
   }
   return x;
 }
