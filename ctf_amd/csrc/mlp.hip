@@ -65,13 +65,21 @@ __global__ void k_swizzle(const float *__restrict__ th, uint16_t *__restrict__ f
   const int lane = e / 4, j = e % 4;
   const int row = lane % 16, k = 4 * (lane / 16) + j;   // A[row][k] of the fragment
   float v;
-  if (isf) {
-    int layer, mt, kt;
-    if (f < MLP_F2) { layer = 1; mt = f; kt = 0; }
-    else if (f < MLP_F3) { layer = 2; mt = (f - MLP_F2) / 4; kt = (f - MLP_F2) % 4; }
-    else if (f < MLP_F4) { layer = 3; mt = (f - MLP_F3) / 4; kt = (f - MLP_F3) % 4; }
-    else { layer = 4; mt = 0; kt = f - MLP_F4; }
-    v = weight(th, layer, 16 * mt + row, 16 * kt + k);
+  if (isf && f < MLP_F2) {
+    v = weight(th, 1, 16 * f + row, k);
+    fwd[u] = f32_to_bf16(v);
+  }
+  else if (isf) {
+    // 16x32 tiles of the double-k product: lane l2, slot s holds A[l2 % 16][k2] with
+    // k2 = 4 (l2 / 16) + s for s < 4, 16 + 4 (l2 / 16) + (s - 4) for s >= 4
+    const int layer = f < MLP_F3 ? 2 : f < MLP_F4 ? 3 : 4;
+    const int base = layer == 2 ? MLP_F2 : layer == 3 ? MLP_F3 : MLP_F4;
+    const int uu = u - base * MLP_FRAG;
+    const int q = uu / (2 * MLP_FRAG), e2 = uu % (2 * MLP_FRAG);
+    const int l2 = e2 / 8, s = e2 % 8;
+    const int mt = layer == 4 ? 0 : q / 2, kt = layer == 4 ? q : q % 2;
+    const int k2 = 32 * kt + (s < 4 ? 4 * (l2 / 16) + s : 16 + 4 * (l2 / 16) + (s - 4));
+    v = weight(th, layer, 16 * mt + l2 % 16, k2);
     fwd[u] = f32_to_bf16(v);
   }
   else {

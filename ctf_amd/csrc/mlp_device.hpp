@@ -1,5 +1,7 @@
 /* Learned conserved-to-primitive initial guess: a 6 -> 64 -> 64 -> 64 -> 2 MLP
- * evaluated with BF16 matrix cores (v_mfma_f32_16x16x16_bf16) and FP32 accumulation.
+ * evaluated with BF16 matrix cores and FP32 accumulation: v_mfma_f32_16x16x16_bf16 for the
+ * first layer (16 input columns) and gfx950's v_mfma_f32_16x16x32_bf16, twice the k per
+ * instruction, for the three products over the 64 hidden features.
  *
  * Activation: hard-swish, a * clamp(a/6 + 1/2, 0, 1) (Howard et al. 2019, "Searching for
  * MobileNetV3"), the piecewise-quadratic form of SiLU a * sigmoid(a). It costs two
@@ -11,8 +13,15 @@
  * so that the MFMA result layout (lane l holds rows 4*(l/16)+r, column l%16 of a 16x16
  * tile) is exactly the B-operand layout the next layer needs (lane l holds k = 4*(l/16)+r
  * of column l%16): activations stay in registers from layer to layer. Weights are stored
- * pre-swizzled in "fragment order" (per 16x16 tile, 64 lanes x 4 bf16), so a lane loads
- * its A operand with one 8-byte read.
+ * pre-swizzled in "fragment order" (per 16x16 tile, 64 lanes x 4 bf16; per 16x32 tile of
+ * the double-k product, 64 lanes x 8 bf16), so a lane loads its A operand with one 8- or
+ * 16-byte read.
+ * The double-k product pairs slot s of lane group g = l/16 in A with the same slot and group
+ * in B, whatever k the hardware gives that pair, so a fixed assignment of the 32 k of a tile
+ * to (g, s) gives the product as long as the A fragments are stored with it. Here
+ * k = 4g + s for s < 4 and 16 + 4g + (s - 4) for s >= 4: a lane's B operand for k-tile t is
+ * then the two activation fragments it already holds for m-tiles 2t and 2t + 1 of the
+ * previous layer.
  *
  * Features of a conserved state U = (D, tau, S, B) (all scale-free; D normalises):
  *   x0 = log(tau/D), x1 = log(1 + |S|^2/D^2), x2 = log(1 + |B|^2/D),
@@ -41,8 +50,9 @@ constexpr int MLP_OUT = 2;      // outputs
 constexpr int MLP_MOUT = 16;    // padded output width (one MFMA m-tile)
 constexpr int MLP_FRAG = 256;   // bf16 per 16x16 fragment (64 lanes x 4)
 
-// forward (A = W^T) fragments: layer 1: 4 m-tiles x 1 k-tile; layers 2, 3: 4 x 4;
-// layer 4: 1 x 4
+// forward (A = W^T) fragments, in MLP_FRAG units: layer 1: 4 m-tiles x 1 k-tile of 16;
+// layers 2, 3: 4 m-tiles x 2 k-tiles of 32 (two units each, tile (mt, kt) at 2 (2 mt + kt));
+// layer 4: 1 x 2 k-tiles of 32
 constexpr int MLP_F1 = 0, MLP_F2 = 4, MLP_F3 = 20, MLP_F4 = 36, MLP_NFWD = 40;
 // backward (A = W) fragments for the hidden-state gradients: layer 4: 4 m-tiles (in) x
 // 1 k-tile (out); layers 3, 2: 4 x 4
@@ -127,6 +137,29 @@ __device__ __forceinline__ bf16x4_t load_frag(const uint16_t *frags, int f, int 
   r[2] = (short)(v.y & 0xffff);
   r[3] = (short)(v.y >> 16);
   return r;
+}
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef short s16x8_t __attribute__((ext_vector_type(8)));
+
+/* gfx950's 16x16x32 BF16 product (k assignment in the header comment). */
+__device__ __forceinline__ f32x4_t mfma32(bf16x8_t a, bf16x8_t b, f32x4_t c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+/* A 16x32 fragment: two consecutive MLP_FRAG units starting at unit f. */
+__device__ __forceinline__ bf16x8_t load_frag32(const uint16_t *frags, int f, int lane)
+{
+  const uint4 v = *reinterpret_cast<const uint4 *>(frags + (size_t)f * MLP_FRAG + lane * 8);
+  return __builtin_bit_cast(bf16x8_t, v);
+}
+
+/* The B operand of k-tile t from the fragments of m-tiles 2t (slots 0-3) and 2t + 1. */
+__device__ __forceinline__ bf16x8_t cat8(bf16x4_t lo, bf16x4_t hi)
+{
+  const s16x8_t v = __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
+  return __builtin_bit_cast(bf16x8_t, v);
 }
 
 __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d)
@@ -214,10 +247,11 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t *xin, const uint
 #pragma unroll
       for (int mt = 0; mt < 4; ++mt) {
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) {
-          const bf16x4_t a = load_frag(wf, f0 + 4 * mt + kt, lane);
+        for (int kt = 0; kt < 2; ++kt) {
+          const bf16x8_t a = load_frag32(wf, f0 + 2 * (2 * mt + kt), lane);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) C[mt][j] = mfma16(a, H[kt][2 * half + j], C[mt][j]);
+          for (int j = 0; j < 2; ++j)
+            C[mt][j] = mfma32(a, cat8(H[2 * kt][2 * half + j], H[2 * kt + 1][2 * half + j]), C[mt][j]);
         }
       }
 #pragma unroll
@@ -241,7 +275,8 @@ __device__ __forceinline__ void mlp_forward_wave(const uint16_t *xin, const uint
   for (int nt = 0; nt < 4; ++nt) {
     f32x4_t c = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) c = mfma16(load_frag(wf, MLP_F4 + kt, lane), H[kt][nt], c);
+    for (int kt = 0; kt < 2; ++kt)
+      c = mfma32(load_frag32(wf, MLP_F4 + 2 * kt, lane), cat8(H[2 * kt][nt], H[2 * kt + 1][nt]), c);
     if (grp == 0) {
       yout[(16 * nt + col) * 2 + 0] = c[0] + b4[0];
       yout[(16 * nt + col) * 2 + 1] = c[1] + b4[1];
