@@ -69,6 +69,17 @@ def sweep_flops(w, fc=None):
     return (nif[0] * fpi["x"] + nif[1] * fpi["y"] + nif[2] * fpi["z"]) / 3.0
 
 
+def sweep_hw_ops(w):
+    """Hardware FP32 operations of one FP32 sweep launch (mean over the axes): the count per
+    interface measured at 256^3 times the interfaces of the sweep; None without the file."""
+    path = os.path.join(ROOT, "profiles", "sweep_ops_c2_256.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        per = json.load(f)["fp32_ops_per_interface"]
+    return per * sum(interfaces_per_sweep(w)) / 3.0
+
+
 def stage_count(w):
     return {"single": 1, "rk2": 2, "rk3": 3, "rk4": 4}[w.advance]
 
@@ -107,14 +118,23 @@ def form_roofline(w, r, form):
     f_sweep = sweep_flops(w, fc)
     t_sweep = r["flux_ms_per_launch"] * 1e-3
     peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
-    ach = f_sweep / t_sweep / 1e12
+    # The FP64 sweep is priced by the oracle's count along its path. The FP32 sweep accepts
+    # the HLLD start without a Newton step at most interfaces, so that count would credit it
+    # with work it skips: it is priced by the hardware FP32 operations it issues per
+    # interface (rocprofv3 SQ counters at 256^3, profiles/sweep_ops_c2_256.json).
+    hw = sweep_hw_ops(w) if fp32 else None
+    f_priced = hw if hw is not None else f_sweep
+    ach = f_priced / t_sweep / 1e12
     sweep = {"kernel": f"k_flux_hlld_fast<{'float' if fp32 else 'double'}> + k_flux_redo (one axis sweep)",
              "bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-             "flops_per_launch": f_sweep, "avg_launch_ms": r["flux_ms_per_launch"],
-             "flops_source": "oracle count along the device's path (tests/golden/flops_*.json device_path)",
+             "flops_per_launch": f_priced, "avg_launch_ms": r["flux_ms_per_launch"],
+             "flops_source": ("hardware FP32 operations per interface (FMA = 2) x interfaces, "
+                              "profiles/sweep_ops_c2_256.json" if hw is not None else
+                              "oracle count along the device's path (tests/golden/flops_*.json device_path)"),
              "bytes_per_launch": nif * (64 + (32 if fp32 else 64)),
              "hbm_GBs": nif * (64 + (32 if fp32 else 64)) / t_sweep / 1e9}
     if fp32:
+        sweep["oracle_flops_per_launch"] = f_sweep
         sweep["peak_without_packed_fp32"] = FP32_UNPACKED_TFLOPS
         sweep["frac_without_packed_fp32"] = ach / FP32_UNPACKED_TFLOPS
     chained = form != "plain"
@@ -130,21 +150,22 @@ def form_roofline(w, r, form):
            "frac": pb / t_p2c / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": pb, "avg_launch_ms": r["c2p_ms_per_launch"]}
     stages = stage_count(w)
     c2p_zone = fc["flops_per_c2p_zone"]
-    step_flops = (3 * stages * f_sweep + (stages + 1) * nint * c2p_zone + stages * nint * UPDATE_FLOPS_PER_ZONE)
+    step_flops = (3 * stages * f_priced + (stages + 1) * nint * c2p_zone + stages * nint * UPDATE_FLOPS_PER_ZONE)
     step_s = r["ms_per_step"] * 1e-3
     step = {"flops_per_step": step_flops, "achieved": step_flops / step_s / 1e12, "peak": peak,
             "unit": "TFLOP/s", "frac": step_flops / step_s / 1e12 / peak,
-            "what": "the step's algorithmic flops (sweeps, c2p passes at the reference cascade's count, "
-                    "updates) over its wall time"}
+            "what": "the step's flops (sweeps as priced above, c2p passes at the reference cascade's "
+                    "count, updates) over its wall time"}
     out = {"sweep": sweep, "update_c2p": upd, "p2c_c2p": p2c, "step": step}
     if form == "learned":
-        mf = MLP_FLOPS_PER_ZONE * nint * (stages + 1)
-        t_l = (r["update_ms_per_launch"] * stages + r["c2p_ms_per_launch"]) * 1e-3
-        out["network"] = {"bound": "mfma", "achieved": mf / t_l / 1e12, "peak": BF16_PEAK_TFLOPS,
-                          "unit": "TFLOP/s", "frac": mf / t_l / 1e12 / BF16_PEAK_TFLOPS,
+        # the network runs once per step, in the step-start pass
+        mf = MLP_FLOPS_PER_ZONE * nint
+        out["network"] = {"bound": "mfma", "achieved": mf / t_p2c / 1e12, "peak": BF16_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": mf / t_p2c / 1e12 / BF16_PEAK_TFLOPS,
                           "flops_per_step": mf,
-                          "what": "the network's matrix-core flops over the time of the kernels it runs in "
-                                  "(which also do the update and the Newton polish)"}
+                          "what": "the network's matrix-core flops over the time of the step-start kernel "
+                                  "it runs in (k_p2c_c2p<C2P_NET>, which also does PrimToCons and the "
+                                  "Newton polish)"}
     return out
 
 
