@@ -670,6 +670,53 @@ __device__ __forceinline__ bool sweep_interface(const GridDesc &g, long &z)
   }
 }
 
+/* Block order of the stage update. A zone's update reads the flux planes at its own
+ * index and at +-1 in i, j and k (the constrained-transport averages), so in the flat order
+ * the i - 1 and i + 1 planes come back from beyond L2 (an i-plane of the three flux arrays
+ * is 6.3 MB in FP32 at 256^3). As in the sweeps, XCD x takes a contiguous eighth of the
+ * logical blocks, and the logical order runs in columns of kUpdateChunk tiles (a tile is a
+ * block's 256 consecutive zones of an i-plane) over all i-planes before the next column
+ * starts. Every zone is updated once, whatever the order. Measured against the flat order:
+ * 512^3 update 44.5 -> 42.5 ms per plain step and 30.0 -> 28.6 mixed; 256^3 unchanged
+ * (plain) to 1.5 % slower (mixed); columns of 4 or 64 tiles, or the XCD split alone, were
+ * slower. */
+constexpr unsigned kUpdateChunk = 16;
+
+/* Blocks of the update's grid: N0 planes of ceil(N1 N2 / kBlock) tiles. */
+inline unsigned update_blocks(const int *N)
+{
+  const long plane = (long)N[1] * N[2];
+  return (unsigned)(N[0] * ((plane + kBlock - 1) / kBlock));
+}
+
+/* The calling thread's interior zone t (flat (i, j, k) order, 0 <= t < N0 N1 N2); false
+ * where it has none (the last tile of a plane). */
+__device__ __forceinline__ bool update_zone_index(const GridDesc &g, long &t)
+{
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned q = nb >> 3;
+  const unsigned L = b < 8 * q ? (b & 7) * q + (b >> 3) : b;
+  const unsigned plane = (unsigned)g.N[1] * (unsigned)g.N[2];
+  const unsigned ntiles = (plane + kBlock - 1) / kBlock;
+  const unsigned nA = (unsigned)g.N[0];
+  constexpr unsigned C = kUpdateChunk;
+  const unsigned full = ntiles / C, per_full = nA * C;
+  unsigned a, tile;
+  if (L < full * per_full) {
+    const unsigned rem = L % per_full;
+    a = rem / C;
+    tile = (L / per_full) * C + rem % C;
+  }
+  else {
+    const unsigned width = ntiles - full * C, rem = L - full * per_full;
+    a = rem / width;
+    tile = full * C + rem % width;
+  }
+  const unsigned r = tile * kBlock + threadIdx.x;
+  t = (long)a * plane + r;
+  return r < plane;
+}
+
 /* HLLD sweep along DIM, common path only (interface_flux_fast): interfaces that need
  * the c2p cascade, a later Newton attempt, the small-normal-field branch, the HLLC
  * fallback or the retry are queued for k_flux_redo, which computes them with the full
@@ -1373,13 +1420,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_update_c2p(
 {
   static_assert(MODE == C2P_CASCADE || MODE == C2P_STORED, "stage passes start from the cascade or a stored start");
   constexpr int NC = 4;
-  const long ni = (long)g.N[0] * g.N[1] * g.N[2];
   const long T = g.T;
   unsigned long long cnt[NC] = {};
   {
-    const long base = (long)blockIdx.x * kBlock;
-    const long t = base + threadIdx.x;
-    const bool valid = t < ni;
+    long t = 0;
+    const bool valid = update_zone_index(g, t);
     int i = 0, j = 0, k = 0;
     long z = 0;
     double u[8], p[8];
@@ -2249,8 +2294,10 @@ int maragpu_create(const maragpu_config *cfg, maragpu_ctx **out)
   if (hipMalloc(&c->dstats, sizeof(DevStats)) != hipSuccess) return fail_free(c);
   if (hipHostMalloc(&c->hstats, sizeof(DevStats), hipHostMallocDefault) != hipSuccess) return fail_free(c);
   if (hipMalloc(&c->ddiv, 4 * sizeof(double)) != hipSuccess) return fail_free(c);
-  if (hipMalloc(&c->c2p_part, (size_t)nblocks((long)cfg->N[0] * cfg->N[1] * cfg->N[2]) * (kBlock / 64) *
-                                  sizeof(uint4)) != hipSuccess)
+  // per-wave counter slots of the c2p passes: the update's grid, or the flat grid of the
+  // step start, whichever is larger
+  const unsigned part_blocks = std::max(update_blocks(cfg->N), nblocks((long)cfg->N[0] * cfg->N[1] * cfg->N[2]));
+  if (hipMalloc(&c->c2p_part, (size_t)part_blocks * (kBlock / 64) * sizeof(uint4)) != hipSuccess)
     return fail_free(c);
   if ((cfg->c2p == MARAGPU_C2P_LEARNED || cfg->precision == MARAGPU_MIXED) &&
       hipMalloc(&c->start, (size_t)cfg->N[0] * cfg->N[1] * cfg->N[2] * sizeof(float2)) != hipSuccess)
@@ -2521,7 +2568,7 @@ int maragpu_advance(maragpu_ctx *c, double dt, maragpu_stats *stats)
     join_redo();
     EventPair p = mark(eu);
     hipEventRecord(c->ev[p.a], c->stream);
-    const unsigned nb = nblocks(ni);
+    const unsigned nb = update_blocks(g.N);
 #define UPDATE_ARGS(FT) (const FT *)c->F[0], (const FT *)c->F[1], (const FT *)c->F[2], c->U0, c->U, c->A, c->Pw, \
     g, dt, kind, write_u, c->cfg.gamma, c->cfg.pfloor, c->cfg.ereset, c->cfg.preset, c->cfg.dreset, c->dstats, pass, c->Uw, \
     c->start, c->c2p_part
