@@ -224,6 +224,31 @@ def test_turbulence_steps_vs_oracle(gpu, oracle_lib, extrap, riemann, advance, N
     compare_states(ctx.download(), Po)
 
 
+def test_noncubic_turbulence_vs_oracle(gpu, oracle_lib):
+    """A grid with three different sizes (20 x 12 x 36 zones: i-planes of 432 zones, so the
+    update's block tiles end part-way through a plane, and the sweeps' block ranges end
+    part-way through a tile column), WENO5/HLLD/RK3 for 4 steps against the oracle with the
+    plain-form tolerances."""
+    N = (20, 12, 36)
+    P0 = gpu.init_turbulence(N, v_rms=0.4, v_kmax=4.0, four_velocity=True, dB_rms=0.3,
+                             A_kmax=3.0, seed=11)
+    s = Setup(N=N, extrap="weno5", riemann="hlld", advance="rk3")
+    so = oracle_lib.solver(s)
+    ctx = gpu.Context(setup_to_config(gpu, s))
+    ctx.upload(P0)
+    Po = P0.copy()
+    dt = 0.0
+    for _ in range(4):
+        eo, sto = so.advance(Po, dt)
+        eg, stg = ctx.advance(dt)
+        assert eo == eg == 0
+        assert abs(stg.max_lambda - sto.max_lambda) <= 1e-12 * sto.max_lambda
+        assert abs(stg.hlld_success - sto.hlld_success) <= 2
+        dt = so.get_timestep(0.4)
+        assert abs(ctx.get_timestep(0.4) - dt) <= 1e-12 * dt
+    compare_states(ctx.download(), Po)
+
+
 @pytest.mark.parametrize("eigen", ["component", "rmhd"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_weno_split_steps_vs_oracle(gpu, oracle_lib, precision, eigen):
