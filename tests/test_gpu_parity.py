@@ -224,15 +224,22 @@ def test_turbulence_steps_vs_oracle(gpu, oracle_lib, extrap, riemann, advance, N
     compare_states(ctx.download(), Po)
 
 
-def test_noncubic_turbulence_vs_oracle(gpu, oracle_lib):
+@pytest.mark.parametrize("op", ["plm-split", "plm-muscl", "weno-split-component", "weno-split-rmhd"])
+def test_noncubic_turbulence_vs_oracle(gpu, oracle_lib, op):
     """A grid with three different sizes (20 x 12 x 36 zones: i-planes of 432 zones, so the
     update's block tiles end part-way through a plane, and the sweeps' block ranges end
-    part-way through a tile column), WENO5/HLLD/RK3 for 4 steps against the oracle with the
-    plain-form tolerances."""
+    part-way through a tile column), each operator with RK3 for 4 steps against the oracle
+    with the plain-form tolerances: plm-split (WENO5/HLLD), plm-muscl (HLLD) and weno-split
+    with both projections."""
     N = (20, 12, 36)
     P0 = gpu.init_turbulence(N, v_rms=0.4, v_kmax=4.0, four_velocity=True, dB_rms=0.3,
                              A_kmax=3.0, seed=11)
-    s = Setup(N=N, extrap="weno5", riemann="hlld", advance="rk3")
+    if op == "plm-split":
+        s = Setup(N=N, extrap="weno5", riemann="hlld", advance="rk3")
+    elif op == "plm-muscl":
+        s = Setup(N=N, godunov="plm-muscl", riemann="hlld", advance="rk3")
+    else:
+        s = Setup(N=N, godunov="weno-split", advance="rk3", eigen=op.split("-")[-1])
     so = oracle_lib.solver(s)
     ctx = gpu.Context(setup_to_config(gpu, s))
     ctx.upload(P0)
