@@ -54,17 +54,18 @@ def test_mixed_riemann_close_to_fp64(gpu, solver):
         assert np.mean(rel < 1e-2) >= 0.98, (dim, np.sort(rel)[-5:])
 
 
-@pytest.mark.parametrize("extrap,advance", [("weno5", "rk3"), ("plm", "rk2")])
-def test_mixed_steps_close_to_the_oracle(gpu, oracle_lib, extrap, advance):
+@pytest.mark.parametrize("extrap,advance,riemann", [("weno5", "rk3", "hlld"), ("plm", "rk2", "hlld"),
+                                                    ("plm", "rk2", "hll")])
+def test_mixed_steps_close_to_the_oracle(gpu, oracle_lib, extrap, advance, riemann):
     N, steps = 24, 6
     P0 = gpu.init_turbulence((N, N, N), v_rms=0.5, v_kmax=6.0, four_velocity=True, dB_rms=0.5,
                              A_kmax=4.0, seed=31)
-    so = oracle_lib.solver(Setup(N=(N, N, N), extrap=extrap, riemann="hlld", advance=advance))
-    ctx = gpu.Context(gpu.make_config(N=(N, N, N), extrap=extrap, riemann="hlld", advance=advance,
+    so = oracle_lib.solver(Setup(N=(N, N, N), extrap=extrap, riemann=riemann, advance=advance))
+    ctx = gpu.Context(gpu.make_config(N=(N, N, N), extrap=extrap, riemann=riemann, advance=advance,
                                       precision="mixed"))
     ctx.upload(P0)
     Po = P0.copy()
-    ref = gpu.Context(gpu.make_config(N=(N, N, N), extrap=extrap, riemann="hlld", advance=advance))
+    ref = gpu.Context(gpu.make_config(N=(N, N, N), extrap=extrap, riemann=riemann, advance=advance))
     ref.upload(P0)
     dt = 0.0
     for _ in range(steps):
