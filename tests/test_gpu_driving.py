@@ -38,14 +38,14 @@ def test_sampled_field_vs_oracle(gpu, oracle_lib):
     assert (np.abs(Fd - Fo) <= 1e-13 * amp[None, :]).all(), np.abs(Fd - Fo).max(axis=0) / amp
 
 
+@pytest.mark.parametrize("N", [(16, 16, 16), (12, 20, 8)], ids=["16^3", "12x20x8"])
 @pytest.mark.parametrize("cool", [None, "T4", "E4"])
-def test_driven_cooled_steps_vs_oracle(gpu, oracle_lib, cool):
-    N = 16
-    s = Setup(N=(N, N, N), extrap="plm", riemann="hlld", advance="rk2")
+def test_driven_cooled_steps_vs_oracle(gpu, oracle_lib, cool, N):
+    s = Setup(N=N, extrap="plm", riemann="hlld", advance="rk2")
     so = oracle_lib.solver(s)
-    P0 = gpu.init_turbulence((N, N, N), v_rms=0.3, four_velocity=True, dB_rms=0.3, seed=21)
+    P0 = gpu.init_turbulence(N, v_rms=0.3, four_velocity=True, dB_rms=0.3, seed=21)
     f = gpu.OuField.create(2.0, 0.3, 2, 77)
-    ctx = gpu.Context(gpu.make_config(N=(N, N, N), extrap="plm", riemann="hlld", advance="rk2"))
+    ctx = gpu.Context(gpu.make_config(N=N, extrap="plm", riemann="hlld", advance="rk2"))
     ctx.upload(P0)
     ref = 1e-18 if cool == "T4" else 0.4
     ctx.set_cooling(cool, ref, 50.0)
