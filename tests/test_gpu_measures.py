@@ -34,11 +34,11 @@ def check(dev, ref):
     assert (np.abs(a[means] - b[means]) <= 1e-13 * scale + 1e-15).all(), np.abs(a[means] - b[means]) / scale
 
 
+@pytest.mark.parametrize("N", [(20, 20, 20), (14, 22, 10)], ids=["20^3", "14x22x10"])
 @pytest.mark.parametrize("units", [None, (3.0, 5.0, 0.5)])
-def test_measures_vs_oracle(gpu, oracle_lib, units):
-    N = 20
-    P0 = gpu.init_turbulence((N, N, N), v_rms=0.5, four_velocity=True, dB_rms=0.4, A_kmax=3.0, seed=12)
-    ctx = gpu.Context(gpu.make_config(N=(N, N, N), extrap="plm", riemann="hlld", advance="rk2"))
+def test_measures_vs_oracle(gpu, oracle_lib, units, N):
+    P0 = gpu.init_turbulence(N, v_rms=0.5, four_velocity=True, dB_rms=0.4, A_kmax=3.0, seed=12)
+    ctx = gpu.Context(gpu.make_config(N=N, extrap="plm", riemann="hlld", advance="rk2"))
     ctx.upload(P0)
     check(ctx.measure(units), oracle_lib.measures(P0, gamma=G, units=units or (1.0, 1.0, 1.0)))
     # after two steps: the committed state
