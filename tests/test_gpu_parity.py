@@ -256,6 +256,35 @@ def test_noncubic_turbulence_vs_oracle(gpu, oracle_lib, op):
     compare_states(ctx.download(), Po)
 
 
+@pytest.mark.parametrize("Ng,extrap", [(2, "plm"), (4, "weno5")])
+def test_ghost_count_vs_oracle(gpu, oracle_lib, Ng, extrap):
+    """Two and four ghost zones (Mara's domain takes any count the stencil allows): the flat
+    sweep range, the update's stencil and the ghost images all follow Ng. FP64 against the
+    oracle with the plain-form tolerances, ghosts included."""
+    N = (16, 12, 20)
+    P0 = gpu.init_turbulence(N, Ng=Ng, v_rms=0.4, v_kmax=4.0, four_velocity=True, dB_rms=0.3,
+                             A_kmax=3.0, seed=5)
+    s = Setup(N=N, Ng=Ng, extrap=extrap, riemann="hlld", advance="rk2")
+    so = oracle_lib.solver(s)
+    ctx = gpu.Context(setup_to_config(gpu, s))
+    ctx.upload(P0)
+    Po = P0.copy()
+    dt = 0.0
+    for _ in range(3):
+        eo, sto = so.advance(Po, dt)
+        eg, stg = ctx.advance(dt)
+        assert eo == eg == 0
+        assert abs(stg.max_lambda - sto.max_lambda) <= 1e-12 * sto.max_lambda
+        dt = so.get_timestep(0.4)
+    Pg = ctx.download()
+    it = (slice(Ng, -Ng),) * 3
+    a, b = Pg[it], Po[it]
+    assert np.isfinite(a).all()
+    L1 = np.abs(a - b).sum(axis=(0, 1, 2)) / np.maximum(np.abs(b).sum(axis=(0, 1, 2)), 1e-300)
+    assert L1.max() < 1e-10, L1
+    assert np.array_equal(Pg, synchronize(Pg, Ng))
+
+
 @pytest.mark.parametrize("eigen", ["component", "rmhd"])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_weno_split_steps_vs_oracle(gpu, oracle_lib, precision, eigen):
