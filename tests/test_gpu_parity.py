@@ -256,6 +256,37 @@ def test_noncubic_turbulence_vs_oracle(gpu, oracle_lib, op):
     compare_states(ctx.download(), Po)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gamma_and_domain_vs_oracle(gpu, oracle_lib, precision):
+    """Gamma = 5/3 on the domain [-0.5, 0.5] x [-1, 1] x [0, 2] (cells twice as long in y and
+    z as in x), WENO5/HLLD/RK3 for 3 steps: FP64 with the plain-form tolerances, mixed with
+    L1 <= 1e-6."""
+    N = (16, 16, 16)
+    P0 = gpu.init_turbulence(N, v_rms=0.4, v_kmax=4.0, four_velocity=True, dB_rms=0.3,
+                             A_kmax=3.0, seed=9)
+    s = Setup(N=N, gamma=5.0 / 3.0, x0=(-0.5, -1.0, 0.0), x1=(0.5, 1.0, 2.0), extrap="weno5",
+              riemann="hlld", advance="rk3")
+    so = oracle_lib.solver(s)
+    ctx = gpu.Context(setup_to_config(gpu, s, precision=precision))
+    ctx.upload(P0)
+    Po = P0.copy()
+    dt = 0.0
+    for _ in range(3):
+        eo, sto = so.advance(Po, dt)
+        eg, stg = ctx.advance(dt)
+        assert eo == eg == 0
+        tol = 1e-12 if precision == "fp64" else 1e-4
+        assert abs(stg.max_lambda - sto.max_lambda) <= tol * sto.max_lambda
+        dt = so.get_timestep(0.4)
+    if precision == "fp64":
+        compare_states(ctx.download(), Po)
+    else:
+        Pg = ctx.download()
+        it = (slice(3, -3),) * 3
+        L1 = np.abs(Pg[it] - Po[it]).sum(axis=(0, 1, 2)) / np.abs(Po[it]).sum(axis=(0, 1, 2))
+        assert L1.max() <= 1e-6, L1
+
+
 @pytest.mark.parametrize("Ng,extrap", [(2, "plm"), (4, "weno5")])
 def test_ghost_count_vs_oracle(gpu, oracle_lib, Ng, extrap):
     """Two and four ghost zones (Mara's domain takes any count the stencil allows): the flat
