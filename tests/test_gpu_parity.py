@@ -195,6 +195,38 @@ def test_full_steps_golden(gpu, name):
         assert np.array_equal(Pg[3:-3, 3:-3, 3:-3], g["P0"])
 
 
+@pytest.mark.parametrize("name", ["plm_hlld_rk2_superluminal_ereset", "plm_hlld_rk2_cold_pfloor",
+                                  "plm_hlld_rk2_jump", "weno5_hlld_rk3"])
+def test_full_steps_golden_mixed(gpu, oracle_lib, name):
+    """The mixed form on reference-written grids with an emergency reset, a pressure floor and
+    a jump, on the reference's own dt sequence: the same error codes, and every field within
+    L1 1e-5 of the reference's state (FP32 fluxes; the c2p, its floor and its resets stay
+    FP64). The pressure of a cold flow is the small difference of the energy and its kinetic
+    and magnetic parts, so an FP32 rounding of the energy flux (1e-7 of tau) is 1e-7 tau/p of
+    p: its bound is 1e-7 sum|tau| / sum|p| where that exceeds 1e-5 (340 times at the
+    pressure floor's grid)."""
+    import ast
+    g = load_golden(f"grid_{name}.npz")
+    s = Setup(**ast.literal_eval(str(g["setup"])))
+    ctx = gpu.Context(setup_to_config(gpu, s, precision="mixed"))
+    ctx.upload(np.ascontiguousarray(embed(g["P0"])))
+    for k, dt in enumerate(g["dts"]):
+        e, st = ctx.advance(float(dt))
+        assert e == g["errors"][k]
+        assert st.fp64_redo_overflow == 0
+    Pg = ctx.download()
+    assert np.isfinite(Pg).all()
+    if g["errors"][-1] == 0:
+        it = (slice(3, -3),) * 3
+        a, b = Pg[it], g["P"]
+        L1 = np.abs(a - b).sum(axis=(0, 1, 2)) / np.maximum(np.abs(b).sum(axis=(0, 1, 2)), 1e-300)
+        U, _ = oracle_lib.prim_to_cons(b.reshape(-1, 8), s.gamma)
+        bound = np.full(8, 1e-5)
+        bound[1] = max(1e-5, 1e-7 * np.abs(U[:, 1]).sum() / np.abs(b[..., 1]).sum())
+        assert (L1 <= bound).all(), (L1, bound)
+        assert np.array_equal(Pg, synchronize(Pg, 3))
+
+
 @pytest.mark.parametrize("extrap,riemann,advance,N", [
     ("plm", "hlld", "rk2", 32), ("weno5", "hlld", "rk3", 24), ("weno5", "hllc", "rk3", 16),
     ("plm", "hll", "rk4", 16)])
