@@ -195,20 +195,26 @@ def test_full_steps_golden(gpu, name):
         assert np.array_equal(Pg[3:-3, 3:-3, 3:-3], g["P0"])
 
 
+@pytest.mark.parametrize("c2p", ["plain", "learned"])
 @pytest.mark.parametrize("name", ["plm_hlld_rk2_superluminal_ereset", "plm_hlld_rk2_cold_pfloor",
                                   "plm_hlld_rk2_jump", "weno5_hlld_rk3"])
-def test_full_steps_golden_mixed(gpu, oracle_lib, name):
+def test_full_steps_golden_mixed(gpu, oracle_lib, name, c2p):
     """The mixed form on reference-written grids with an emergency reset, a pressure floor and
     a jump, on the reference's own dt sequence: the same error codes, and every field within
     L1 1e-5 of the reference's state (FP32 fluxes; the c2p, its floor and its resets stay
     FP64). The pressure of a cold flow is the small difference of the energy and its kinetic
     and magnetic parts, so an FP32 rounding of the energy flux (1e-7 of tau) is 1e-7 tau/p of
     p: its bound is 1e-7 sum|tau| / sum|p| where that exceeds 1e-5 (340 times at the
-    pressure floor's grid)."""
+    pressure floor's grid). With the learned start (the network's guess, then the plain
+    cascade where the Newton polish from it fails) the same holds."""
     import ast
+    import os
     g = load_golden(f"grid_{name}.npz")
     s = Setup(**ast.literal_eval(str(g["setup"])))
-    ctx = gpu.Context(setup_to_config(gpu, s, precision="mixed"))
+    ctx = gpu.Context(setup_to_config(gpu, s, precision="mixed", c2p=c2p))
+    if c2p == "learned":
+        ctx.attach_mlp(gpu.Mlp.load(os.path.join(os.path.dirname(__file__), "..", "ctf_amd", "models",
+                                                 "c2p_mlp.bin")))
     ctx.upload(np.ascontiguousarray(embed(g["P0"])))
     for k, dt in enumerate(g["dts"]):
         e, st = ctx.advance(float(dt))
