@@ -325,15 +325,22 @@ def test_gamma_and_domain_vs_oracle(gpu, oracle_lib, precision):
         assert L1.max() <= 1e-6, L1
 
 
-@pytest.mark.parametrize("Ng,extrap", [(2, "plm"), (4, "weno5")])
-def test_ghost_count_vs_oracle(gpu, oracle_lib, Ng, extrap):
+@pytest.mark.parametrize("Ng,op", [(2, "plm-split"), (4, "plm-split"), (2, "plm-muscl"),
+                                   (4, "weno-split")])
+def test_ghost_count_vs_oracle(gpu, oracle_lib, Ng, op):
     """Two and four ghost zones (Mara's domain takes any count the stencil allows): the flat
-    sweep range, the update's stencil and the ghost images all follow Ng. FP64 against the
-    oracle with the plain-form tolerances, ghosts included."""
+    sweep ranges, the update's stencil and the ghost images all follow Ng. FP64 against the
+    oracle with the plain-form tolerances, ghosts included: plm-split (PLM with 2, WENO5 with
+    4), plm-muscl with 2, weno-split (characteristic) with 4."""
     N = (16, 12, 20)
     P0 = gpu.init_turbulence(N, Ng=Ng, v_rms=0.4, v_kmax=4.0, four_velocity=True, dB_rms=0.3,
                              A_kmax=3.0, seed=5)
-    s = Setup(N=N, Ng=Ng, extrap=extrap, riemann="hlld", advance="rk2")
+    if op == "plm-split":
+        s = Setup(N=N, Ng=Ng, extrap="plm" if Ng == 2 else "weno5", riemann="hlld", advance="rk2")
+    elif op == "plm-muscl":
+        s = Setup(N=N, Ng=Ng, godunov="plm-muscl", riemann="hlld", advance="rk2")
+    else:
+        s = Setup(N=N, Ng=Ng, godunov="weno-split", eigen="rmhd", advance="rk2")
     so = oracle_lib.solver(s)
     ctx = gpu.Context(setup_to_config(gpu, s))
     ctx.upload(P0)
