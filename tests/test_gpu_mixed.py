@@ -55,7 +55,8 @@ def test_mixed_riemann_close_to_fp64(gpu, solver):
 
 
 @pytest.mark.parametrize("extrap,advance,riemann", [("weno5", "rk3", "hlld"), ("plm", "rk2", "hlld"),
-                                                    ("plm", "rk2", "hll")])
+                                                    ("plm", "rk2", "hll"), ("plm", "single", "hlld"),
+                                                    ("plm", "rk4", "hlld")])
 def test_mixed_steps_close_to_the_oracle(gpu, oracle_lib, extrap, advance, riemann):
     N, steps = 24, 6
     P0 = gpu.init_turbulence((N, N, N), v_rms=0.5, v_kmax=6.0, four_velocity=True, dB_rms=0.5,
